@@ -1,0 +1,247 @@
+"""Python host interface over libwfdp.so.
+
+Two layers:
+
+* the reference's operator API, one function per problem with the
+  reference's argument meaning -- ``LCS(n1, n2, arr1, arr2, paddX, paddY,
+  table)`` (LCS/GPU_Hyperlane_Share.h:5), ``SW(..., tX, tY)``
+  (SmithWaterman/GPU_Hyperlane_Share.h:5), ``ED(..., tX, tY)``
+  (EditDistance/GPU_Hyperlane_Share.h:5) -- returning the bottom-right cell;
+* contexts, batches and plans for the batched and served workloads.
+
+Every call goes through the HIP library; nothing here computes a DP cell.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import WfError, check
+
+PROBLEM_IDS = {"LCS": _lib.WF_LCS, "ED": _lib.WF_ED, "SW": _lib.WF_SW}
+
+
+def _problem_id(problem) -> int:
+    if isinstance(problem, str):
+        return PROBLEM_IDS[problem.upper()]
+    p = int(problem)
+    if p not in (0, 1, 2):
+        raise ValueError(f"unknown problem {problem!r}")
+    return p
+
+
+def _as_i32(x) -> np.ndarray:
+    arr = np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1))
+    return arr
+
+
+def _ptr(arr: np.ndarray):
+    return arr.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+@dataclass
+class Result:
+    last: int        # bottom-right cell: what the reference programs print
+    max_score: int   # SW maximum cell (local alignment score); 0 for LCS/ED
+    status: int
+    kernel_ms: float
+    cells: int
+
+    @classmethod
+    def _from(cls, r: _lib.wf_result) -> "Result":
+        return cls(int(r.last), int(r.max_score), int(r.status), float(r.kernel_ms), int(r.cells))
+
+
+@dataclass
+class Tile:
+    """Tile request: tile_w = steps between boundary exchanges, strip_rows =
+    rows swept by one workgroup, rows_per_lane = DP rows per SIMD lane.
+    Zero means "choose"."""
+    tile_w: int = 0
+    strip_rows: int = 0
+    rows_per_lane: int = 0
+    ed_textbook: bool = False
+    cache: bool = False
+
+    def _c(self) -> _lib.wf_tile:
+        flags = (_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
+        return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
+
+
+@dataclass
+class Job:
+    problem: int | str
+    a: Sequence[int] | np.ndarray
+    b: Sequence[int] | np.ndarray
+    ed_textbook: bool = False
+
+
+class _JobArray:
+    """Keeps the int32 arrays alive while C holds pointers into them."""
+
+    def __init__(self, jobs: Iterable[Job]):
+        self.arrays = []
+        jl = list(jobs)
+        self.c = (_lib.wf_job * max(1, len(jl)))()
+        for i, j in enumerate(jl):
+            a, b = _as_i32(j.a), _as_i32(j.b)
+            self.arrays += [a, b]
+            self.c[i] = _lib.wf_job(_problem_id(j.problem), _lib.WF_TILE_ED_TEXTBOOK if j.ed_textbook else 0,
+                                    _ptr(a), a.size, _ptr(b), b.size)
+        self.n = len(jl)
+
+
+class Plan:
+    """Encoded pairs resident in HBM; ``run`` re-executes the DP."""
+
+    def __init__(self, ctx: "Context", jobs: Iterable[Job], tile: Tile | None = None):
+        self.ctx = ctx
+        self._jobs = _JobArray(jobs)
+        self._h = C.c_void_p()
+        t = tile._c() if tile else None
+        check(ctx._lib.wf_plan_create(ctx._h, self._jobs.c, self._jobs.n, C.byref(t) if t else None,
+                                      C.byref(self._h)), "wf_plan_create")
+        self.n = self._jobs.n
+
+    def run(self, per_pair: bool = False) -> float:
+        check(self.ctx._lib.wf_plan_run(self._h, _lib.WF_RUN_PER_PAIR if per_pair else _lib.WF_RUN_PERSISTENT),
+              "wf_plan_run")
+        return self.kernel_ms
+
+    @property
+    def kernel_ms(self) -> float:
+        return float(self.ctx._lib.wf_plan_kernel_ms(self._h))
+
+    def results(self, raise_on_error: bool = True) -> list[Result]:
+        out = (_lib.wf_result * max(1, self.n))()
+        st = self.ctx._lib.wf_plan_results(self._h, out, self.n)
+        if raise_on_error:
+            check(st, "wf_plan_results")
+        return [Result._from(out[i]) for i in range(self.n)]
+
+    def describe(self) -> dict:
+        r, w, k, g = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        nt = C.c_int64()
+        check(self.ctx._lib.wf_plan_describe(self._h, C.byref(r), C.byref(w), C.byref(k), C.byref(nt), C.byref(g)))
+        return {"rows_per_lane": r.value, "waves": w.value, "chunk": k.value, "tasks": nt.value, "grid": g.value}
+
+    def close(self) -> None:
+        if self._h:
+            self.ctx._lib.wf_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (stream, arenas, events)."""
+
+    def __init__(self, device: int = 0, timeout_s: float | None = None):
+        self._lib = _lib.load()
+        self._h = C.c_void_p()
+        check(self._lib.wf_ctx_create(device, C.byref(self._h)), "wf_ctx_create")
+        if timeout_s is not None:
+            check(self._lib.wf_ctx_set_timeout(self._h, float(timeout_s)))
+
+    def info(self) -> dict:
+        d = _lib.wf_device_info()
+        check(self._lib.wf_device_query(self._h, C.byref(d)))
+        return {"compute_units": d.compute_units, "max_clock_mhz": d.max_clock_mhz,
+                "simds_per_cu": d.simds_per_cu, "wave_size": d.wave_size, "total_mem": d.total_mem,
+                "free_mem": d.free_mem, "arch": d.arch.decode()}
+
+    def align(self, problem, a, b, tile: Tile | None = None, ed_textbook: bool = False) -> Result:
+        a, b = _as_i32(a), _as_i32(b)
+        t = tile._c() if tile else _lib.wf_tile(0, 0, 0, 0)
+        if ed_textbook:
+            t.flags |= _lib.WF_TILE_ED_TEXTBOOK
+        out = _lib.wf_result()
+        check(self._lib.wf_align(self._h, _problem_id(problem), _ptr(a), a.size, _ptr(b), b.size, C.byref(t),
+                                 C.byref(out)), "wf_align")
+        return Result._from(out)
+
+    def align_batch(self, jobs: Iterable[Job], tile: Tile | None = None) -> list[Result]:
+        ja = _JobArray(jobs)
+        out = (_lib.wf_result * max(1, ja.n))()
+        t = tile._c() if tile else None
+        check(self._lib.wf_align_batch(self._h, ja.c, ja.n, C.byref(t) if t else None, out), "wf_align_batch")
+        return [Result._from(out[i]) for i in range(ja.n)]
+
+    def plan(self, jobs: Iterable[Job], tile: Tile | None = None) -> Plan:
+        return Plan(self, jobs, tile)
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.wf_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default: Context | None = None
+
+
+def default_context() -> Context:
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default
+
+
+def _tile_from_ref(tX: int | None, tY: int | None, cache: bool = False) -> Tile:
+    # tY rows per strip and tX columns per tile, as on the reference's argv
+    # (SmithWaterman/GPU_Hyperlane_Share.cpp:56-59).
+    return Tile(tile_w=int(tX or 0), strip_rows=int(tY or 0), cache=cache)
+
+
+def LCS(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
+        tX: int | None = 32, tY: int | None = 256, cache: bool = False) -> int:
+    """Reference signature LCS(n1, n2, arr1, arr2, paddX, paddY, table)
+    (LCS/GPU_Hyperlane_Share.h:5); tile defaults to the hard-coded 32x256 of
+    LCS/GPU_Hyperlane_Share.cu:703-704.  Returns the LCS length."""
+    del paddX, paddY, table  # host table is never written by the reference outside DEBUG builds
+    a, b = _as_i32(arr1)[:n1], _as_i32(arr2)[:n2]
+    return default_context().align("LCS", a, b, _tile_from_ref(tX, tY, cache)).last
+
+
+def SW(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
+       tX: int | None = None, tY: int | None = None, cache: bool = False) -> int:
+    """Reference signature SW(n1, n2, arr1, arr2, paddX, paddY, table, tX, tY)
+    (SmithWaterman/GPU_Hyperlane_Share.h:5).  Returns the bottom-right cell
+    ("last", SmithWaterman/SW.cpp:80)."""
+    del paddX, paddY, table
+    a, b = _as_i32(arr1)[:n1], _as_i32(arr2)[:n2]
+    return default_context().align("SW", a, b, _tile_from_ref(tX, tY, cache)).last
+
+
+def ED(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
+       tX: int | None = None, tY: int | None = None, cache: bool = False, textbook: bool = False) -> int:
+    """Reference signature ED(n1, n2, arr1, arr2, paddX, paddY, table, tX, tY)
+    (EditDistance/GPU_Hyperlane_Share.h:5).  Boundary of EditDistance/ED.cpp
+    (D(0,x) = x-1) unless textbook=True (the reference GPU's D(0,x) = x)."""
+    del paddX, paddY, table
+    a, b = _as_i32(arr1)[:n1], _as_i32(arr2)[:n2]
+    t = _tile_from_ref(tX, tY, cache)
+    t.ed_textbook = textbook
+    return default_context().align("ED", a, b, t).last
+
+
+__all__ = ["Context", "Plan", "Job", "Tile", "Result", "WfError", "LCS", "SW", "ED", "default_context"]

@@ -1,0 +1,613 @@
+// wf_api.cpp -- host side of libwfdp.so: contexts, symbol encoding, strip
+// planning, kernel selection and launches behind the C ABI in
+// include/wfdp.h.
+//
+// The reference's host functions (LCS/GPU_Hyperlane_Share.cu:696-792,
+// SmithWaterman/GPU_Hyperlane_Share.cu:683-781,
+// EditDistance/GPU_Hyperlane_Share.cu:689-789) cudaMalloc and cudaMemset the
+// whole (n1+3) x (n2+1) table, create multiProcessorCount streams and launch
+// one kernel per strip on every call.  Here a context owns one stream and
+// grow-only device arenas; a call encodes the two sequences to one-byte
+// codes, uploads them, and runs one persistent launch whose workgroups claim
+// strips in dependency order.  Device memory per pair is the codes plus one
+// boundary row per strip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/wfdp.h"
+#include "wf_device.h"
+#include "wf_internal.h"
+#include "wf_variants.h"
+
+namespace {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap) return WF_OK;
+        if (ptr) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        want = (want + 255) & ~size_t(255);
+        if (hipMalloc(&ptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            ptr = nullptr;
+            return WF_ERR_NOMEM;
+        }
+        cap = want;
+        return WF_OK;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct WfArenas {
+    DevBuf codes, pairs, tasks, bnd, flags, res, counters;
+    void release()
+    {
+        codes.release();
+        pairs.release();
+        tasks.release();
+        bnd.release();
+        flags.release();
+        res.release();
+        counters.release();
+    }
+};
+
+struct wf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipDeviceProp_t prop{};
+    double timeout_s = 10.0;
+    WfArenas scratch;  // used by wf_align / wf_align_batch
+};
+
+struct wf_plan {
+    wf_ctx *ctx = nullptr;
+    WfArenas own;
+    WfPlanHost host;
+    double kernel_ms = 0.0;
+    bool ran = false;
+};
+
+// ----------------------------------------------------------------------------
+// Encoding
+// ----------------------------------------------------------------------------
+
+// Maps the symbols of one pair to one-byte codes.  Equality is all the three
+// recurrences look at, so any injective renaming gives identical tables.
+// Row tables need codes < 8 (SW, 4-bit fields) or < 32 (LCS, ED, 1-bit
+// fields); larger alphabets fall back to comparing codes.
+int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
+                   uint8_t *out_a, uint8_t *out_b, int32_t *mode)
+{
+    const int table_limit = (problem == WF_SW) ? 8 : 32;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t i = 0; i < n1; ++i) {
+        lo = std::min(lo, a[i]);
+        hi = std::max(hi, a[i]);
+    }
+    for (int64_t i = 0; i < n2; ++i) {
+        lo = std::min(lo, b[i]);
+        hi = std::max(hi, b[i]);
+    }
+    const int scale = (problem == WF_SW) ? 4 : 1;  // SW codes are bit offsets of 4-bit fields
+    *mode = 0;
+    if (n1 + n2 == 0) return WF_OK;
+    if (lo >= 0 && hi < table_limit) {
+        for (int64_t i = 0; i < n1; ++i) out_a[i] = (uint8_t)(a[i] * scale);
+        for (int64_t i = 0; i < n2; ++i) out_b[i] = (uint8_t)(b[i] * scale);
+        return WF_OK;
+    }
+    // Dense renaming in order of first appearance.
+    std::unordered_map<int32_t, int> ids;
+    ids.reserve(64);
+    auto id_of = [&](int32_t v) -> int {
+        auto it = ids.find(v);
+        if (it != ids.end()) return it->second;
+        const int id = (int)ids.size();
+        ids.emplace(v, id);
+        return id;
+    };
+    std::vector<int> ia(n1), ib(n2);
+    for (int64_t i = 0; i < n1; ++i) ia[i] = id_of(a[i]);
+    for (int64_t i = 0; i < n2; ++i) ib[i] = id_of(b[i]);
+    const int nsym = (int)ids.size();
+    if (nsym > 256) return WF_ERR_ALPHABET;
+    if (nsym <= table_limit) {
+        for (int64_t i = 0; i < n1; ++i) out_a[i] = (uint8_t)(ia[i] * scale);
+        for (int64_t i = 0; i < n2; ++i) out_b[i] = (uint8_t)(ib[i] * scale);
+    } else {
+        *mode |= WFK_MODE_CMP;
+        for (int64_t i = 0; i < n1; ++i) out_a[i] = (uint8_t)ia[i];
+        for (int64_t i = 0; i < n2; ++i) out_b[i] = (uint8_t)ib[i];
+    }
+    return WF_OK;
+}
+
+// Scores of degenerate pairs (an empty sequence), per the reference
+// boundaries: LCS/SW 0; ED D(n2,0) = n2, D(0,n1) = n1-1 (n1 >= 1, ED.cpp
+// :48-54) or n1 (textbook).
+static int32_t empty_score(int problem, int64_t n1, int64_t n2, int flags)
+{
+    if (problem != WF_ED) return 0;
+    if (n2 == 0) {
+        if (n1 == 0) return 0;
+        return (flags & WF_TILE_ED_TEXTBOOK) ? (int32_t)n1 : (int32_t)(n1 - 1);
+    }
+    return (int32_t)n2;
+}
+
+// ----------------------------------------------------------------------------
+// Variant choice
+// ----------------------------------------------------------------------------
+
+static const WfVariant *find_variant(int R, int W, int K)
+{
+    for (int i = 0; i < wf_num_variants; ++i)
+        if (wf_variants[i].rows_per_lane == R && wf_variants[i].waves == W && wf_variants[i].chunk == K)
+            return &wf_variants[i];
+    return nullptr;
+}
+
+// Nearest compiled variant to a requested strip height / chunk width.
+const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t total_cells,
+                                 int npairs)
+{
+    int R = 0, W = 0, K = 0;
+    if (const char *env = std::getenv("WFDP_VARIANT")) {
+        if (std::sscanf(env, "%d,%d,%d", &R, &W, &K) == 3) {
+            if (const WfVariant *v = find_variant(R, W, K)) return v;
+        }
+        R = W = K = 0;
+    }
+    const bool cache = tile && (tile->flags & WF_TILE_CACHE);
+    if (tile && tile->rows_per_lane > 0) R = tile->rows_per_lane;
+    if (tile && tile->tile_w > 0) K = tile->tile_w <= 16 ? 16 : (tile->tile_w <= 32 ? 32 : 64);
+    if (tile && tile->strip_rows > 0) {
+        const int rows = tile->strip_rows;
+        if (cache) W = 1;
+        if (R == 0 && W == 0) {
+            // Prefer 4 waves (one per SIMD), then fewer rows per lane.
+            if (rows >= 256) {
+                W = 4;
+                R = std::max(1, std::min(4, rows / 256));
+            } else if (rows >= 128) {
+                W = 2;
+                R = 1;
+            } else {
+                W = 1;
+                R = 1;
+            }
+        } else if (W == 0) {
+            W = std::max(1, rows / (64 * R));
+        } else if (R == 0) {
+            R = std::max(1, rows / (64 * W));
+        }
+    }
+    if (W == 0) W = cache ? 1 : wf_default_waves(max_rows, total_cells, npairs);
+    if (R == 0) R = wf_default_rows_per_lane(W, max_rows, total_cells, npairs);
+    if (K == 0) K = (W == 1) ? 32 : (W >= 8 ? 16 : 16);
+    // Snap to the nearest compiled variant.
+    const WfVariant *best = nullptr;
+    int best_d = 1 << 30;
+    for (int i = 0; i < wf_num_variants; ++i) {
+        const WfVariant &v = wf_variants[i];
+        const int d = std::abs(v.waves - W) * 1000 + std::abs(v.rows_per_lane - R) * 10 +
+                      std::abs(v.chunk - K) / 16;
+        if (d < best_d) {
+            best_d = d;
+            best = &v;
+        }
+    }
+    return best;
+}
+
+int wf_default_waves(int64_t max_rows, int64_t total_cells, int npairs)
+{
+    (void)total_cells;
+    if (npairs == 1 && max_rows >= 256) return 4;
+    return max_rows >= 1024 ? 4 : 1;
+}
+
+int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, int npairs)
+{
+    (void)total_cells;
+    (void)npairs;
+    if (waves >= 4) return max_rows >= 65536 ? 2 : 1;
+    return max_rows >= 512 ? 2 : 1;
+}
+
+// ----------------------------------------------------------------------------
+// Planning
+// ----------------------------------------------------------------------------
+
+int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
+                  WfPlanHost *ph)
+{
+    (void)ctx;
+    ph->npairs = njobs;
+    ph->pairs.assign(njobs, WfPairDev{});
+    ph->host_result.assign(njobs, 0);
+    ph->host_only.assign(njobs, 0);
+    ph->cells.assign(njobs, 0);
+    int64_t max_rows = 0, total_cells = 0;
+    for (int64_t i = 0; i < njobs; ++i) {
+        const wf_job &j = jobs[i];
+        if (j.problem < WF_LCS || j.problem > WF_SW) return WF_ERR_INVALID;
+        if (j.n1 < 0 || j.n2 < 0 || j.n1 > (1LL << 30) || j.n2 > (1LL << 30)) return WF_ERR_INVALID;
+        if ((j.n1 > 0 && !j.a) || (j.n2 > 0 && !j.b)) return WF_ERR_INVALID;
+        if (j.n1 > 0 && j.n2 > 0) max_rows = std::max(max_rows, j.n2);
+        total_cells += j.n1 * j.n2;
+        ph->cells[i] = (uint64_t)(j.n1 * j.n2);
+    }
+    const WfVariant *v = wf_pick_variant(tile, max_rows, total_cells, (int)njobs);
+    if (!v) return WF_ERR_NOKERNEL;
+    ph->variant = v;
+    const int64_t strip = 64LL * v->waves * v->rows_per_lane;
+
+    // Codes: a then b per pair, 16-byte aligned.
+    int64_t code_bytes = 0, bnd_ints = 0, nflags = 0;
+    for (int64_t i = 0; i < njobs; ++i) {
+        const wf_job &j = jobs[i];
+        WfPairDev &p = ph->pairs[i];
+        p.prob = j.problem;
+        p.n1 = (int32_t)j.n1;
+        p.n2 = (int32_t)j.n2;
+        p.a_off = code_bytes;
+        code_bytes += (j.n1 + 15) & ~15LL;
+        p.b_off = code_bytes;
+        code_bytes += (j.n2 + 15) & ~15LL;
+        if (j.n1 == 0 || j.n2 == 0) {
+            ph->host_only[i] = 1;
+            ph->host_result[i] = empty_score(j.problem, j.n1, j.n2, j.flags);
+            p.ngroups = 0;
+            continue;
+        }
+        p.ngroups = (int32_t)((j.n2 + strip - 1) / strip);
+        p.bnd_off = bnd_ints;
+        bnd_ints += (int64_t)(p.ngroups - 1) * j.n1;
+        p.flag_off = (int32_t)nflags;
+        nflags += p.ngroups;
+    }
+    ph->codes.assign((size_t)std::max<int64_t>(code_bytes, 16), 0);
+    for (int64_t i = 0; i < njobs; ++i) {
+        const wf_job &j = jobs[i];
+        WfPairDev &p = ph->pairs[i];
+        int32_t mode = 0;
+        const int st = wf_encode_pair(j.problem, j.a, j.n1, j.b, j.n2, ph->codes.data() + p.a_off,
+                                      ph->codes.data() + p.b_off, &mode);
+        if (st != WF_OK) return st;
+        if (j.problem == WF_ED && (j.flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
+        if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
+        p.mode = mode;
+    }
+
+    // Tasks: largest pairs first; the strips of a pair stay contiguous and in
+    // order, which is what makes the persistent claim loop deadlock-free.
+    std::vector<int64_t> order(njobs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        return ph->cells[x] > ph->cells[y];
+    });
+    ph->tasks.clear();
+    ph->task_begin.assign(njobs, 0);
+    ph->task_end.assign(njobs, 0);
+    for (int64_t k = 0; k < njobs; ++k) {
+        const int64_t i = order[k];
+        ph->task_begin[i] = (int64_t)ph->tasks.size();
+        for (int32_t g = 0; g < ph->pairs[i].ngroups; ++g) ph->tasks.push_back(WfTaskDev{(int32_t)i, g});
+        ph->task_end[i] = (int64_t)ph->tasks.size();
+    }
+    ph->bnd_ints = bnd_ints;
+    ph->nflags = nflags;
+    ph->order = order;
+    return WF_OK;
+}
+
+static int upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
+{
+    const size_t ntasks = ph->tasks.size();
+    int st;
+    if ((st = ar->codes.ensure(ph->codes.size())) != WF_OK) return st;
+    if ((st = ar->pairs.ensure(std::max<size_t>(1, ph->pairs.size()) * sizeof(WfPairDev))) != WF_OK) return st;
+    if ((st = ar->tasks.ensure(std::max<size_t>(1, ntasks) * sizeof(WfTaskDev))) != WF_OK) return st;
+    if ((st = ar->bnd.ensure(std::max<int64_t>(1, ph->bnd_ints) * sizeof(int32_t))) != WF_OK) return st;
+    if ((st = ar->flags.ensure(std::max<int64_t>(1, ph->nflags) * sizeof(int32_t))) != WF_OK) return st;
+    if ((st = ar->res.ensure(std::max<int64_t>(1, ph->npairs) * sizeof(WfResultDev))) != WF_OK) return st;
+    if ((st = ar->counters.ensure((size_t)(ph->npairs + 1) * sizeof(int32_t))) != WF_OK) return st;
+    if (hipMemcpyAsync(ar->codes.ptr, ph->codes.data(), ph->codes.size(), hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (!ph->pairs.empty() &&
+        hipMemcpyAsync(ar->pairs.ptr, ph->pairs.data(), ph->pairs.size() * sizeof(WfPairDev),
+                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (ntasks &&
+        hipMemcpyAsync(ar->tasks.ptr, ph->tasks.data(), ntasks * sizeof(WfTaskDev),
+                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    return WF_OK;
+}
+
+static int max_resident_groups(wf_ctx *ctx, const WfVariant *v)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v->fn, 64 * v->waves, 0) != hipSuccess ||
+        per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    if (const char *env = std::getenv("WFDP_WG_PER_CU")) {
+        const int e = std::atoi(env);
+        if (e > 0) per_cu = std::min(per_cu, e);
+    }
+    return per_cu * ctx->prop.multiProcessorCount;
+}
+
+// Enqueue reset + engine launches and time the engine part.
+static int run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *kernel_ms)
+{
+    const WfVariant *v = ph->variant;
+    const int64_t ntasks = (int64_t)ph->tasks.size();
+    const int64_t ncounters = ph->npairs + 1;
+    if (wf_launch_reset((int32_t *)ar->flags.ptr, ph->nflags, (WfResultDev *)ar->res.ptr, ph->npairs,
+                        (int32_t *)ar->counters.ptr, ncounters, ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    WfKernelArgs ka{};
+    ka.pairs = (const WfPairDev *)ar->pairs.ptr;
+    ka.tasks = (const WfTaskDev *)ar->tasks.ptr;
+    ka.codes = (const uint8_t *)ar->codes.ptr;
+    ka.bnd = (int32_t *)ar->bnd.ptr;
+    ka.flags = (int32_t *)ar->flags.ptr;
+    ka.res = (WfResultDev *)ar->res.ptr;
+    ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
+    const int maxg = max_resident_groups(ctx, v);
+    ph->grid = 0;
+    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (ntasks > 0) {
+        if (mode == WF_RUN_PER_PAIR) {
+            for (int64_t k = 0; k < ph->npairs; ++k) {
+                const int64_t i = ph->order[k];
+                const int64_t tb = ph->task_begin[i], te = ph->task_end[i];
+                if (te <= tb) continue;
+                ka.counter = (int32_t *)ar->counters.ptr + 1 + i;
+                ka.task_begin = (int32_t)tb;
+                ka.task_end = (int32_t)te;
+                const int grid = (int)std::min<int64_t>(te - tb, maxg);
+                ph->grid = std::max(ph->grid, grid);
+                if (wf_launch_engine(v, grid, &ka, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+            }
+        } else {
+            ka.counter = (int32_t *)ar->counters.ptr;
+            ka.task_begin = 0;
+            ka.task_end = (int32_t)ntasks;
+            const int grid = (int)std::min<int64_t>(ntasks, maxg);
+            ph->grid = grid;
+            if (wf_launch_engine(v, grid, &ka, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+        }
+    }
+    if (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (hipEventSynchronize(ctx->ev1) != hipSuccess) return WF_ERR_DEVICE;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) return WF_ERR_DEVICE;
+    *kernel_ms = ms;
+    return WF_OK;
+}
+
+static int download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms,
+                            wf_result *out, int64_t n)
+{
+    std::vector<WfResultDev> r((size_t)std::max<int64_t>(1, ph->npairs));
+    if (ph->npairs > 0 &&
+        hipMemcpy(r.data(), ar->res.ptr, ph->npairs * sizeof(WfResultDev), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return WF_ERR_DEVICE;
+    (void)ctx;
+    int worst = WF_OK;
+    for (int64_t i = 0; i < std::min(n, ph->npairs); ++i) {
+        wf_result &o = out[i];
+        std::memset(&o, 0, sizeof(o));
+        o.cells = ph->cells[i];
+        o.kernel_ms = kernel_ms;
+        if (ph->host_only[i]) {
+            o.last = ph->host_result[i];
+            o.max_score = 0;
+            continue;
+        }
+        o.last = r[i].last;
+        o.max_score = r[i].maxs;
+        o.status = r[i].status ? WF_ERR_TIMEOUT : WF_OK;
+        if (o.status) worst = o.status;
+    }
+    return worst;
+}
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+
+extern "C" {
+
+int wf_abi_version(void) { return WFDP_ABI_VERSION; }
+
+const char *wf_strerror(int status)
+{
+    switch (status) {
+    case WF_OK: return "ok";
+    case WF_ERR_INVALID: return "invalid argument";
+    case WF_ERR_DEVICE: return "HIP runtime error";
+    case WF_ERR_NOMEM: return "device memory exhausted";
+    case WF_ERR_ALPHABET: return "more than 256 distinct symbols in one pair";
+    case WF_ERR_TIMEOUT: return "device-side dependency wait timed out";
+    case WF_ERR_NOKERNEL: return "no compiled kernel for the requested tile";
+    case WF_ERR_AGAIN: return "queue full, retry";
+    default: return "unknown error";
+    }
+}
+
+int wf_ctx_create(int device, wf_ctx **out)
+{
+    if (!out) return WF_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return WF_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) return WF_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return WF_ERR_DEVICE;
+    wf_ctx *c = new wf_ctx();
+    c->device = device;
+    if (hipGetDeviceProperties(&c->prop, device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return WF_ERR_DEVICE;
+    }
+    *out = c;
+    return WF_OK;
+}
+
+void wf_ctx_destroy(wf_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int wf_ctx_set_timeout(wf_ctx *ctx, double seconds)
+{
+    if (!ctx || !(seconds > 0.0)) return WF_ERR_INVALID;
+    ctx->timeout_s = seconds;
+    return WF_OK;
+}
+
+int wf_device_query(wf_ctx *ctx, wf_device_info *out)
+{
+    if (!ctx || !out) return WF_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->compute_units = ctx->prop.multiProcessorCount;
+    out->max_clock_mhz = ctx->prop.clockRate / 1000;
+    out->simds_per_cu = 4;
+    out->wave_size = ctx->prop.warpSize;
+    (void)hipSetDevice(ctx->device);
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+        out->free_mem = (int64_t)fr;
+        out->total_mem = (int64_t)tot;
+    }
+    std::snprintf(out->arch, sizeof(out->arch), "%s", ctx->prop.gcnArchName);
+    return WF_OK;
+}
+
+int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile, wf_result *out)
+{
+    if (!ctx || (njobs > 0 && (!jobs || !out)) || njobs < 0) return WF_ERR_INVALID;
+    if (njobs == 0) return WF_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    WfPlanHost ph;
+    int st = wf_build_plan(ctx, jobs, njobs, tile, &ph);
+    if (st != WF_OK) return st;
+    if ((st = upload_plan(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
+    double ms = 0.0;
+    if ((st = run_plan(ctx, &ph, &ctx->scratch, WF_RUN_PERSISTENT, &ms)) != WF_OK) return st;
+    return download_results(ctx, &ph, &ctx->scratch, ms, out, njobs);
+}
+
+int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
+             const wf_tile *tile, wf_result *out)
+{
+    if (!ctx || !out) return WF_ERR_INVALID;
+    wf_job j{problem, tile ? (tile->flags & WF_TILE_ED_TEXTBOOK) : 0, a, n1, b, n2};
+    return wf_align_batch(ctx, &j, 1, tile, out);
+}
+
+int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile, wf_plan **out)
+{
+    if (!ctx || !out || njobs <= 0 || !jobs) return WF_ERR_INVALID;
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    wf_plan *p = new wf_plan();
+    p->ctx = ctx;
+    int st = wf_build_plan(ctx, jobs, njobs, tile, &p->host);
+    if (st == WF_OK) st = upload_plan(ctx, &p->host, &p->own);
+    if (st != WF_OK) {
+        p->own.release();
+        delete p;
+        return st;
+    }
+    *out = p;
+    return WF_OK;
+}
+
+int wf_plan_run(wf_plan *plan, int mode)
+{
+    if (!plan || (mode != WF_RUN_PERSISTENT && mode != WF_RUN_PER_PAIR)) return WF_ERR_INVALID;
+    if (hipSetDevice(plan->ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    const int st = run_plan(plan->ctx, &plan->host, &plan->own, mode, &plan->kernel_ms);
+    plan->ran = (st == WF_OK);
+    return st;
+}
+
+int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n)
+{
+    if (!plan || !out || n < 0) return WF_ERR_INVALID;
+    if (!plan->ran) return WF_ERR_INVALID;
+    if (hipSetDevice(plan->ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    return download_results(plan->ctx, &plan->host, &plan->own, plan->kernel_ms, out, n);
+}
+
+double wf_plan_kernel_ms(const wf_plan *plan) { return plan ? plan->kernel_ms : -1.0; }
+
+int wf_plan_describe(const wf_plan *plan, int32_t *rows_per_lane, int32_t *waves, int32_t *chunk,
+                     int64_t *ntasks, int32_t *grid)
+{
+    if (!plan) return WF_ERR_INVALID;
+    const WfVariant *v = plan->host.variant;
+    if (rows_per_lane) *rows_per_lane = v->rows_per_lane;
+    if (waves) *waves = v->waves;
+    if (chunk) *chunk = v->chunk;
+    if (ntasks) *ntasks = (int64_t)plan->host.tasks.size();
+    if (grid) *grid = plan->host.grid;
+    return WF_OK;
+}
+
+void wf_plan_destroy(wf_plan *plan)
+{
+    if (!plan) return;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    plan->own.release();
+    delete plan;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// wf_device.h -- descriptors shared by the host planner (wf_api.cpp) and the
+// gfx950 wavefront kernels (wf_kernels.hip).  Plain structs, no HIP types.
+#ifndef WF_DEVICE_H
+#define WF_DEVICE_H
+#include <stdint.h>
+
+// Problem ids match include/wfdp.h (WF_LCS, WF_ED, WF_SW).
+enum { WFK_LCS = 0, WFK_ED = 1, WFK_SW = 2 };
+
+// Pair mode bits.
+enum {
+    WFK_MODE_CMP = 1,       // symbol codes are compared (alphabet too large for a row table)
+    WFK_MODE_TEXTBOOK = 2,  // ED: D(0,x) = x instead of ED.cpp's x-1
+};
+
+// One sequence pair as the kernels see it.  x runs over a (n1 columns),
+// y over b (n2 rows).  Codes are one byte per symbol in the code arena.
+struct WfPairDev {
+    int64_t a_off;     // byte offset of a's codes
+    int64_t b_off;     // byte offset of b's codes
+    int64_t bnd_off;   // int offset of this pair's strip-boundary rows
+    int32_t n1, n2;
+    int32_t prob;      // WFK_LCS / WFK_ED / WFK_SW
+    int32_t mode;      // WFK_MODE_* bits
+    int32_t flag_off;  // offset of this pair's per-strip progress counters
+    int32_t ngroups;   // strips of (waves * 64 * rows_per_lane) rows
+};
+
+// One unit of persistent work: strip `group` of pair `pair`.  Tasks of a
+// pair appear in strip order, so a strip only ever waits on a strip that was
+// claimed before it.
+struct WfTaskDev {
+    int32_t pair;
+    int32_t group;
+};
+
+struct WfResultDev {
+    int32_t last;      // bottom-right cell (the score every reference program prints)
+    int32_t maxs;      // SW: maximum cell; 0 otherwise
+    int32_t status;    // 0 ok, else WF_ERR_* bits set on the device
+    int32_t pad;
+};
+
+// Kernel argument block (passed by value).
+struct WfKernelArgs {
+    const WfPairDev *pairs;
+    const WfTaskDev *tasks;
+    const uint8_t *codes;
+    int32_t *bnd;            // strip-boundary rows
+    int32_t *flags;          // per-strip published-column counters
+    WfResultDev *res;
+    int32_t *counter;        // task claim counter for this launch
+    int64_t timeout_ticks;   // s_memrealtime ticks (100 MHz) a dependency wait may take
+    int32_t task_begin;
+    int32_t task_end;
+};
+
+#endif

@@ -1,0 +1,448 @@
+// wf_kernels.hip -- gfx950 wavefront kernels for LCS, edit distance and
+// Smith-Waterman (the reference's hyperlane engine, LCS/GPU_Hyperlane_Share.cu
+// :123-687, SmithWaterman/GPU_Hyperlane_Share.cu:127-674,
+// EditDistance/GPU_Hyperlane_Share.cu:131-680, redesigned for CDNA4).
+//
+// Geometry.  A pair's DP table has n2 rows (b) and n1 columns (a).  Rows are
+// cut into strips of W*64*R rows; one workgroup of W wave64s sweeps a strip
+// left to right.  Inside a wave, lane l owns R consecutive rows and at step s
+// computes column x = s - l of all of them: the lanes form a systolic array
+// whose front is a parallelogram (the reference's "hyperlane" tile), so every
+// lane is busy except in the first and last 63 steps of a strip.  A step
+// passes the lane's bottom cell to the next lane with one DPP wave_shr:1
+// (no LDS, no shuffle); lane 0 takes its upper neighbour from an LDS ring
+// written by the wave above.  The tile body never leaves VGPRs.
+//
+// Waves of a workgroup advance in lock-step chunks of K steps separated by
+// one s_barrier; wave w runs D chunks behind wave w-1, which is exactly the
+// time lane 63 of wave w-1 needs to produce the columns lane 0 of wave w
+// reads.  The strip's bottom row (lane 63 of the last wave) is the only data
+// that goes to HBM: it is copied out every chunk and published with a
+// release store of the number of finished columns; the next strip's first
+// wave polls that counter with an acquire load.  This replaces the
+// reference's one-launch-per-strip + volatile flag scheme
+// (LCS/GPU_Hyperlane_Share.cu:102-121, 741-767) with one persistent launch
+// that claims strips from an atomic counter in dependency order.
+//
+// Cell functions are the reference recurrences rewritten so that each costs
+// 3-4 VALU ops with a per-row lookup table indexed by the column symbol:
+//   LCS  H = max3(up, left, diag + eq)          (LCS/LCS.cpp:58-63)
+//   ED   E = min3(E_up, E_left, E_diag + ne - 2), E = D - x - y
+//                                               (EditDistance/ED.cpp:61-63)
+//   SW   H = usat(max3(up, left, diag + s + 2) - 2)
+//                                               (SmithWaterman/SW.cpp:63-65)
+// The ED and LCS rewrites rely on adjacent cells differing by at most one;
+// the oracle tests check them against the original recurrences.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wf_device.h"
+
+namespace wfk {
+
+constexpr int kDppWaveShr1 = 0x138;  // DPP_CTRL wave_shr:1 (GFX9 family)
+
+constexpr int pow2_at_least(int v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Chunk geometry for chunk width K (steps between barriers).
+template <int K>
+struct ChunkGeom {
+    static_assert(K == 16 || K == 32 || K == 64, "K must divide 64");
+    // Lag in chunks between consecutive waves: lane 0 of wave w reads, at
+    // chunk-aligned groups of 4 columns, values lane 63 of wave w-1 wrote
+    // 63..66 steps later in its own step count.
+    static constexpr int D = (K + 62) / K + 1;
+    // LDS ring of bottom-lane values, indexed by (column + 64) mod RS.
+    static constexpr int RS = pow2_at_least((D + 2) * K);
+};
+
+// ----------------------------------------------------------------------------
+// Cell functions.  `ca` is the column symbol code, `t` the lane's row entry.
+// BFE mode: t is a per-row table, ca a bit offset into it.
+// CMP mode: t is the row symbol code (0x100 for padding rows), ca the code.
+// ----------------------------------------------------------------------------
+template <int P, bool CMP>
+struct Cell;
+
+template <bool CMP>
+struct Cell<WFK_LCS, CMP> {
+    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
+    {
+        if (CMP) return valid ? bc : 0x100u;
+        return valid ? (1u << bc) : 0u;
+    }
+    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
+    {
+        const int eq = CMP ? (int)(ca == t) : (int)__builtin_amdgcn_ubfe(t, ca, 1);
+        return max(max(up, left), diag + eq);
+    }
+};
+
+template <bool CMP>
+struct Cell<WFK_ED, CMP> {
+    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
+    {
+        if (CMP) return valid ? bc : 0x100u;
+        return valid ? ~(1u << bc) : 0xffffffffu;
+    }
+    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
+    {
+        const int ne = CMP ? (int)(ca != t) : (int)__builtin_amdgcn_ubfe(t, ca, 1);
+        return min(min(up, left), diag + ne - 2);
+    }
+};
+
+template <bool CMP>
+struct Cell<WFK_SW, CMP> {
+    // BFE: 4-bit signed field at bit 4*code holds s+2: 5 on a match, -1 else.
+    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
+    {
+        if (CMP) return valid ? bc : 0x100u;
+        return valid ? ((~(0xfu << bc)) | (5u << bc)) : 0xffffffffu;
+    }
+    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
+    {
+        const int s2 = CMP ? ((ca == t) ? 5 : -1) : __builtin_amdgcn_sbfe((int)t, ca, 4);
+        const int m = max(max(up, left), diag + s2);
+        // max(m - 2, 0) for m >= 0: unsigned saturating subtract.
+        return (int)__builtin_elementwise_sub_sat((uint32_t)m, 2u);
+    }
+};
+
+template <int R>
+struct LaneState {
+    int h[R];        // H(row r, last column) -- the next step's "left"
+    int dg;          // H(row above row 0, last column) -- the next step's "diag"
+    int best;        // SW running maximum
+    uint32_t t[R];   // per-row table / code
+};
+
+template <int P, bool CMP, int R, bool MASKED, bool TRACK>
+__device__ __forceinline__ void step(LaneState<R> &st, int inj, uint32_t ca, bool valid)
+{
+    const int up0 = __builtin_amdgcn_update_dpp(inj, st.h[R - 1], kDppWaveShr1, 0xf, 0xf, false);
+    int up = up0;
+    int diag = st.dg;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int left = st.h[r];
+        int h = Cell<P, CMP>::f(up, left, diag, ca, st.t[r]);
+        if (MASKED) {
+            h = valid ? h : left;
+            if (TRACK) st.best = max(st.best, valid ? h : 0);
+        } else if (TRACK) {
+            st.best = max(st.best, h);
+        }
+        diag = left;
+        st.h[r] = h;
+        up = h;
+    }
+    st.dg = MASKED ? (valid ? up0 : st.dg) : up0;
+}
+
+// LDS workspace of one workgroup.
+template <int W, int K, int AW>
+struct WgShared {
+    static constexpr int RS = ChunkGeom<K>::RS;
+    // ring[0]: this strip's top input (lane 0 of wave 0); ring[w+1]: bottom
+    // lane of wave w, read by wave w+1 (and by the copy-out for w = W-1).
+    __attribute__((aligned(16))) int ring[W + 1][RS];
+    // Lanes 0..62 store their bottom cell here so that the capture store is
+    // one unmasked ds_write per step; only lane 63's store lands in a ring.
+    int dummy[W][64 + K];
+    // Window of column codes: column x lives at x mod AW, the first K slots
+    // are mirrored after the end so a chunk never wraps.
+    uint8_t aw[AW + K];
+    int task;
+    // Chunk index at which wave 0 hit a dependency timeout (INT_MAX: none).
+    // Waves test `abort_at <= j` after barrier j; wave 0 can only store a
+    // value > j before the next barrier, so every wave leaves the chunk loop
+    // at the same barrier.
+    int abort_at;
+};
+
+// One chunk of K steps starting at s0 (a multiple of K) for one wave.
+template <int P, bool CMP, int R, int K, int AW, bool MASKED, bool TRACK>
+__device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, int n1,
+                                          const int *ring_prev, int *ring_cur, int *dummy,
+                                          const uint8_t *aw)
+{
+    constexpr int RS = ChunkGeom<K>::RS;
+    const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (RS - 1));
+    const uint8_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
+    int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
+    int *cap_last = (lane == 63) ? ring_cur + ((s0 + K) & (RS - 1)) : dummy + lane + (K - 1);
+    // Software pipeline: the LDS reads of group q+1 (4 column codes and the
+    // 4 top-neighbour values of lane 0) are issued before group q's capture
+    // stores, so their latency hides behind group q's arithmetic.
+    int4 inj_n = *reinterpret_cast<const int4 *>(inj_base);
+    uint32_t ca_n[4] = {cp[0], cp[1], cp[2], cp[3]};
+#pragma unroll
+    for (int q = 0; q < K; q += 4) {
+        const int injv[4] = {inj_n.x, inj_n.y, inj_n.z, inj_n.w};
+        const uint32_t cav[4] = {ca_n[0], ca_n[1], ca_n[2], ca_n[3]};
+        if (q + 4 < K) {
+            inj_n = *reinterpret_cast<const int4 *>(inj_base + q + 4);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ca_n[u] = cp[q + 4 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = q + u;
+            bool valid = true;
+            if (MASKED) valid = (unsigned)(s0 + j - lane) < (unsigned)n1;
+            step<P, CMP, R, MASKED, TRACK>(st, injv[u], cav[u], valid);
+            if (j == K - 1)
+                *cap_last = st.h[R - 1];
+            else
+                cap[j] = st.h[R - 1];
+        }
+    }
+}
+
+__device__ __forceinline__ int64_t now_ticks()
+{
+    return (int64_t)__builtin_amdgcn_s_memrealtime();
+}
+
+// Wave-uniform wait until *flag >= need.  Returns false on timeout.
+__device__ bool wait_flag(const int32_t *flag, int need, int64_t timeout)
+{
+    int v = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    if (v >= need) return true;
+    const int64_t t0 = now_ticks();
+    for (;;) {
+        __builtin_amdgcn_s_sleep(2);
+        v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (v >= need) return true;
+        if (now_ticks() - t0 > timeout) return false;
+    }
+}
+
+// One strip (`g`) of one pair, swept by the whole workgroup.
+template <int P, bool CMP, int R, int W, int K, int AW>
+__device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
+                          WgShared<W, K, AW> &sh, int lane, int wave)
+{
+    constexpr int D = ChunkGeom<K>::D;
+    constexpr int RS = ChunkGeom<K>::RS;
+    constexpr bool TRACK = (P == WFK_SW);
+    const int n1 = pd.n1, n2 = pd.n2;
+    const int64_t row0 = (int64_t)g * (W * 64 * R) + wave * (64 * R) + lane * R;
+    const uint8_t *acodes = ka.codes + pd.a_off;
+    const uint8_t *bcodes = ka.codes + pd.b_off;
+
+    LaneState<R> st;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t y = row0 + r;
+        const bool v = y < n2;
+        st.t[r] = Cell<P, CMP>::row_entry(v ? (uint32_t)bcodes[y] : 0u, v);
+        st.h[r] = 0;
+    }
+    st.dg = 0;
+    st.best = 0;
+
+    const int nsteps = n1 + 63;
+    const int nch = (nsteps + K - 1) / K;
+    const int total = nch + (W - 1) * D;
+    const bool has_prev = g > 0;
+    const bool has_next = g + 1 < pd.ngroups;
+    const int32_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
+    int32_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
+    const int32_t *flag_prev = ka.flags + pd.flag_off + (g - 1);
+    int32_t *flag_cur = ka.flags + pd.flag_off + g;
+    // Row 0 of the table in this kernel's value representation.
+    const int topv = (P == WFK_ED && !(pd.mode & WFK_MODE_TEXTBOOK)) ? -1 : 0;
+
+    // Fill ring[0] and the code window with columns [c*K, c*K + K).
+    auto fill = [&](int c, uint32_t code_pre) -> bool {
+        const int x = c * K + lane;
+        if (has_prev) {
+            const int need = min(n1, (c + 1) * K);
+            if (!wait_flag(flag_prev, need, ka.timeout_ticks)) return false;
+        }
+        if (lane < K && x < n1) {
+            const int v = has_prev ? bprev[x] : topv;
+            sh.ring[0][(x + 64) & (RS - 1)] = v;
+            const int slot = x & (AW - 1);
+            sh.aw[slot] = (uint8_t)code_pre;
+            if (slot < K) sh.aw[AW + slot] = (uint8_t)code_pre;
+        }
+        return true;
+    };
+
+    if (wave == 0) {
+        const uint32_t c0 = (lane < K && lane < n1) ? acodes[lane] : 0u;
+        if (!fill(0, c0) && lane == 0) sh.abort_at = -1;
+    }
+    __syncthreads();
+
+    int copied = 0;  // wave W-1: columns already copied to bnext
+    auto copy_out = [&](int upto) {
+        while (copied < upto) {
+            const int x = copied + lane;
+            if (x < upto) bnext[x] = sh.ring[W][(x + 64) & (RS - 1)];
+            copied = min(upto, copied + 64);
+        }
+        if (lane == 0) __hip_atomic_store(flag_cur, upto, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    };
+
+    bool aborted = sh.abort_at <= -1;
+    for (int j = 0; j < total && !aborted; ++j) {
+        const int cj = j - wave * D;
+        // Wave 0 prefetches the next chunk's column codes from HBM/L2.
+        uint32_t code_pre = 0;
+        if (wave == 0 && j + 1 < nch) {
+            const int x = (j + 1) * K + lane;
+            code_pre = (lane < K && x < n1) ? acodes[x] : 0u;
+        }
+        if (cj >= 0 && cj < nch) {
+            const int s0 = cj * K;
+            if (wave == W - 1 && has_next) {
+                // Steps < s0 are done, so lane 63 has finished columns < s0 - 63.
+                const int upto = min(n1, s0 - 63);
+                if (upto > copied) copy_out(upto);
+            }
+            const bool steady = (s0 >= 63) && (s0 + K - 1 <= n1 - 1);
+            if (steady)
+                run_chunk<P, CMP, R, K, AW, false, TRACK>(st, s0, lane, n1, sh.ring[wave],
+                                                          sh.ring[wave + 1], sh.dummy[wave], sh.aw);
+            else
+                run_chunk<P, CMP, R, K, AW, true, TRACK>(st, s0, lane, n1, sh.ring[wave],
+                                                         sh.ring[wave + 1], sh.dummy[wave], sh.aw);
+        }
+        if (wave == 0 && j + 1 < nch) {
+            if (!fill(j + 1, code_pre) && lane == 0) sh.abort_at = j;
+        }
+        __syncthreads();
+        aborted = sh.abort_at <= j;
+    }
+
+    if (aborted) {
+        // A dependency wait timed out.  Mark the pair and release the strip
+        // below so that it does not wait out its own timeout in turn.
+        if (threadIdx.x == 0) {
+            atomicOr(&ka.res[pair].status, 1);
+            if (has_next)
+                __hip_atomic_store(flag_cur, 0x7fffffff, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    if (wave == W - 1 && has_next) copy_out(n1);
+
+    // Bottom-right cell.
+    const int64_t ylast = (int64_t)n2 - 1;
+    if (ylast >= row0 && ylast < row0 + R) {
+        const int rr = (int)(ylast - row0);
+        int v = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r == rr) v = st.h[r];
+        if (P == WFK_ED) v += n1 + n2;
+        ka.res[pair].last = v;
+    }
+    if (TRACK) {
+        int m = st.best;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+        if (lane == 0 && m > 0) atomicMax(&ka.res[pair].maxs, m);
+    }
+}
+
+template <int R, int W, int K>
+struct EngineCfg {
+    static constexpr int D = ChunkGeom<K>::D;
+    static constexpr int AW = pow2_at_least((2 + (W - 1) * D) * K + 64 + K);
+};
+
+template <int R, int W, int K>
+__global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
+{
+    constexpr int AW = EngineCfg<R, W, K>::AW;
+    __shared__ WgShared<W, K, AW> sh;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sh.task = ka.task_begin + atomicAdd(ka.counter, 1);
+            sh.abort_at = 0x7fffffff;
+        }
+        __syncthreads();
+        const int t = sh.task;
+        __syncthreads();
+        if (t >= ka.task_end) break;
+        const WfTaskDev td = ka.tasks[t];
+        const WfPairDev pd = ka.pairs[td.pair];
+        const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
+        switch (pd.prob * 2 + (cmp ? 1 : 0)) {
+        case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        case WFK_SW * 2 + 0: run_group<WFK_SW, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        default: run_group<WFK_SW, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        }
+        __syncthreads();
+    }
+}
+
+// Zero progress counters, results and claim counters before a launch (a
+// kernel rather than hipMemsetAsync so the sequence can be graph-captured).
+__global__ void wf_reset(int32_t *flags, int64_t nflags, WfResultDev *res, int64_t nres,
+                         int32_t *counters, int64_t ncounters)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = i0; i < nflags; i += stride) flags[i] = 0;
+    for (int64_t i = i0; i < nres; i += stride) res[i] = WfResultDev{0, 0, 0, 0};
+    for (int64_t i = i0; i < ncounters; i += stride) counters[i] = 0;
+}
+
+}  // namespace wfk
+
+// ----------------------------------------------------------------------------
+// Variant table used by the host planner.
+// ----------------------------------------------------------------------------
+#include "wf_variants.h"
+
+#define WF_VARIANT(R_, W_, K_) \
+    {R_, W_, K_, reinterpret_cast<const void *>(&wfk::wf_engine<R_, W_, K_>)}
+
+extern "C" const WfVariant wf_variants[] = {
+    WF_VARIANT(1, 4, 16), WF_VARIANT(2, 4, 16), WF_VARIANT(4, 4, 16),
+    WF_VARIANT(1, 4, 32), WF_VARIANT(2, 4, 32), WF_VARIANT(4, 4, 32),
+    WF_VARIANT(1, 4, 64), WF_VARIANT(2, 4, 64), WF_VARIANT(4, 4, 64),
+    WF_VARIANT(1, 1, 32), WF_VARIANT(2, 1, 32), WF_VARIANT(4, 1, 32), WF_VARIANT(8, 1, 32),
+    WF_VARIANT(1, 8, 16), WF_VARIANT(2, 8, 16),
+    WF_VARIANT(1, 2, 32), WF_VARIANT(2, 2, 32), WF_VARIANT(4, 2, 32),
+};
+extern "C" const int wf_num_variants = sizeof(wf_variants) / sizeof(wf_variants[0]);
+
+extern "C" hipError_t wf_launch_reset(int32_t *flags, int64_t nflags, WfResultDev *res, int64_t nres,
+                                      int32_t *counters, int64_t ncounters, hipStream_t stream)
+{
+    int64_t n = nflags;
+    if (nres > n) n = nres;
+    if (ncounters > n) n = ncounters;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(wfk::wf_reset, dim3(blocks), dim3(256), 0, stream, flags, nflags, res, nres,
+                       counters, ncounters);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
+                                       hipStream_t stream)
+{
+    void *args[] = {const_cast<WfKernelArgs *>(ka)};
+    return hipLaunchKernel(v->fn, dim3(grid), dim3(64 * v->waves), args, 0, stream);
+}

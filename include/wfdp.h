@@ -1,0 +1,139 @@
+/*
+ * wfdp.h -- C ABI of libwfdp.so, the gfx950 wavefront DP engine for LCS,
+ * edit distance and Smith-Waterman.
+ *
+ * The reference exposes one C++ function per problem and per driver, called
+ * from its driver main()s (all paths relative to the reference tree):
+ *
+ *   int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
+ *       LCS/GPU_Hyperlane_Share.h:5, LCS/GPU_Hyperlane_Cache.h:5
+ *   int LCS(int n1, int n2, int *arr1, int *arr2);
+ *       LCS-SHFL/GPU_Hyperlane_Share.h:5, LCS-SHFL/GPU_Hyperlane_Cache.h:5
+ *   int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
+ *       SmithWaterman/GPU_Hyperlane_Share.h:5
+ *   int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
+ *       SmithWaterman/GPU_Hyperlane_Cache.h:5
+ *   int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
+ *       EditDistance/GPU_Hyperlane_Share.h:5
+ *   int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
+ *       EditDistance/GPU_Hyperlane_Cache.h:5
+ *
+ * Each of those allocates the full (n1+3) x (n2+1) table on the device per
+ * call and returns the bottom-right cell.  wf_align() below is the single
+ * entry point those six signatures map onto (wfdp_compat.h gives them back
+ * with their exact C++ signatures); it keeps only strip boundaries in HBM.
+ * Everything else here (contexts, plans, batches, the server) has no
+ * counterpart in the reference and exists for the batched and served
+ * workloads.
+ *
+ * Conventions: x runs over `a` (the reference's arr1, n1 symbols), y over `b`
+ * (arr2, n2 symbols); the returned score is the table cell (n2, n1), exactly
+ * what LCS.cpp / SW.cpp / ED.cpp print for n1 == n2.  Symbols are arbitrary
+ * int32 values compared for equality; at most 256 distinct symbols per pair.
+ * All lengths are 64-bit and may be 0.  Functions return WF_OK or a WF_ERR_*
+ * code instead of exiting (the reference exits: LCS/GPU_Hyperlane_Share.cu
+ * :689-694).  A context is used by one host thread at a time.
+ */
+#ifndef WFDP_H
+#define WFDP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFDP_ABI_VERSION 1
+
+typedef enum { WF_LCS = 0, WF_ED = 1, WF_SW = 2 } wf_problem;
+
+enum {
+    WF_OK = 0,
+    WF_ERR_INVALID = 1,   /* bad argument */
+    WF_ERR_DEVICE = 2,    /* HIP runtime error */
+    WF_ERR_NOMEM = 3,     /* device allocation failed or over the HBM budget */
+    WF_ERR_ALPHABET = 4,  /* more than 256 distinct symbols in one pair */
+    WF_ERR_TIMEOUT = 5,   /* a device-side dependency wait exceeded its limit */
+    WF_ERR_NOKERNEL = 6,  /* no compiled kernel matches the requested tile */
+    WF_ERR_AGAIN = 7      /* server: queue full, retry later */
+};
+
+/* Tile selection.  strip_rows = 64 * waves * rows_per_lane rows are swept by
+ * one workgroup; tile_w = steps between boundary exchanges (16, 32 or 64).
+ * Zero fields mean "choose" (autotuned table, then built-in defaults). */
+typedef struct {
+    int32_t tile_w;
+    int32_t strip_rows;
+    int32_t rows_per_lane;
+    int32_t flags;
+} wf_tile;
+
+#define WF_TILE_ED_TEXTBOOK 0x1 /* ED row 0 = x (reference GPU_setInit), not ED.cpp's x-1 */
+#define WF_TILE_CACHE 0x2       /* hyperlane-Cache policy: one wave per workgroup, all
+                                   strip boundaries through L2 (GPU_Hyperlane_Cache.cu) */
+
+typedef struct {
+    int32_t last;      /* bottom-right cell: the score the reference prints */
+    int32_t max_score; /* SW: maximum cell (local alignment score); 0 for LCS/ED */
+    int32_t status;    /* WF_OK or WF_ERR_* for this pair */
+    int32_t reserved;
+    double kernel_ms;  /* device time of the launch(es) that computed this result */
+    uint64_t cells;    /* n1 * n2 */
+} wf_result;
+
+typedef struct {
+    int32_t problem; /* wf_problem */
+    int32_t flags;   /* WF_TILE_ED_TEXTBOOK */
+    const int32_t *a;
+    int64_t n1;
+    const int32_t *b;
+    int64_t n2;
+} wf_job;
+
+typedef struct {
+    int32_t compute_units;
+    int32_t max_clock_mhz;
+    int32_t simds_per_cu;
+    int32_t wave_size;
+    int64_t total_mem;
+    int64_t free_mem;
+    char arch[32];
+} wf_device_info;
+
+typedef struct wf_ctx wf_ctx;
+typedef struct wf_plan wf_plan;
+
+/* Context: owns a HIP stream, device buffer pools and events on one device. */
+int wf_ctx_create(int device, wf_ctx **out);
+void wf_ctx_destroy(wf_ctx *ctx);
+int wf_device_query(wf_ctx *ctx, wf_device_info *out);
+const char *wf_strerror(int status);
+int wf_abi_version(void);
+/* Seconds a device-side dependency wait may take before the pair fails with
+ * WF_ERR_TIMEOUT (default 10). */
+int wf_ctx_set_timeout(wf_ctx *ctx, double seconds);
+
+/* One pair, synchronous: encode + upload + run + download (the drop-in path
+ * behind the reference's LCS()/SW()/ED()). */
+int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
+             int64_t n2, const wf_tile *tile, wf_result *out);
+
+/* Many pairs, synchronous, one persistent launch. */
+int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
+                   wf_result *out);
+
+/* Plans: inputs encoded and resident in HBM, run any number of times. */
+enum { WF_RUN_PERSISTENT = 0, WF_RUN_PER_PAIR = 1 };
+int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
+                   wf_plan **out);
+int wf_plan_run(wf_plan *plan, int mode);                 /* enqueue + wait */
+int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n);
+double wf_plan_kernel_ms(const wf_plan *plan);            /* last run, HIP events */
+int wf_plan_describe(const wf_plan *plan, int32_t *rows_per_lane, int32_t *waves,
+                     int32_t *chunk, int64_t *ntasks, int32_t *grid);
+void wf_plan_destroy(wf_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WFDP_H */

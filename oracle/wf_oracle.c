@@ -1,0 +1,220 @@
+/*
+ * wf_oracle.c -- CPU restatement of the reference's three wavefront DPs.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product path (the HIP library,
+ * the drivers, the Python host package) links, loads or calls this file.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * it, and only as the checker or the timed CPU baseline.
+ *
+ * Semantics (x runs over a = arr1, n1 symbols; y runs over b = arr2, n2):
+ *
+ *   LCS  LCS/LCS.cpp:45-82
+ *        H(0,.) = H(.,0) = 0                                   (:48-52)
+ *        H(y,x) = a[x-1]==b[y-1] ? H(y-1,x-1)+1
+ *                                : max(H(y,x-1), H(y-1,x))     (:58-63)
+ *        result H(n2,n1)                                       (:79)
+ *   SW   SmithWaterman/SW.cpp:45-83, s(a,b) = a==b ? 3 : -3    (:45-47)
+ *        H(0,.) = H(.,0) = 0                                   (:52-56)
+ *        H(y,x) = max(H(y,x-1)-2, H(y-1,x)-2,
+ *                     H(y-1,x-1)+s(a[x-1],b[y-1]), 0)          (:63-65)
+ *        result "last" = H(n2,n1) (bottom-right, :80); the maximum cell is
+ *        reported separately (the reference does not print it).
+ *   ED   EditDistance/ED.cpp:45-81
+ *        D(0,0) = 0, D(0,x) = x-1 for x >= 1, D(y,0) = y       (:48-54)
+ *        D(y,x) = a[x-1]==b[y-1] ? D(y-1,x-1)
+ *                                : 1+min(D(y-1,x), D(y,x-1), D(y-1,x-1))
+ *                                                              (:61-63)
+ *        result D(n2,n1)                                       (:78)
+ *        The "textbook" flag selects D(0,x) = x instead, which is the
+ *        boundary the reference GPU code writes
+ *        (EditDistance/GPU_Hyperlane_Share.cu:16-22).
+ *
+ * The reference CPU programs index the table as table[y*(n2+1)+x] with the
+ * outer loop over arr2 and the inner over arr1, and are only defined for
+ * n1 == n2 (LCS/LCS.cpp:54-59).  For n1 == n2 the functions here compute the
+ * identical table; for n1 != n2 they follow the GPU reference geometry
+ * (rowsize = 3+n1, x over arr1: LCS/GPU_Hyperlane_Share.cu:705,417-420).
+ *
+ * All indices are 64-bit and only two table rows are kept (rolling rows), so
+ * 2^17 x 2^17 runs in O(n) memory.  The *_table functions keep the whole
+ * (n2+1) x (n1+1) table, in the reference's loop order, for cell-level
+ * comparison and for the full-table timing mode.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "wf_oracle.h"
+
+static inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+static inline int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+
+/* LCS/LCS.cpp:45-82 with rolling rows. */
+int32_t oracle_lcs(const int32_t *a, int64_t n1, const int32_t *b, int64_t n2)
+{
+    if (n1 <= 0 || n2 <= 0) return 0;
+    int32_t *prev = (int32_t *)calloc((size_t)n1 + 1, sizeof(int32_t));
+    int32_t *cur = (int32_t *)calloc((size_t)n1 + 1, sizeof(int32_t));
+    if (!prev || !cur) { free(prev); free(cur); return -1; }
+    for (int64_t y = 1; y <= n2; ++y) {
+        const int32_t by = b[y - 1];
+        cur[0] = 0;
+        for (int64_t x = 1; x <= n1; ++x) {
+            if (a[x - 1] == by)
+                cur[x] = prev[x - 1] + 1;
+            else
+                cur[x] = imax(cur[x - 1], prev[x]);
+        }
+        int32_t *t = prev; prev = cur; cur = t;
+    }
+    int32_t r = prev[n1];
+    free(prev); free(cur);
+    return r;
+}
+
+/* SmithWaterman/SW.cpp:45-83 with rolling rows; *max_out gets max H. */
+int32_t oracle_sw(const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
+                  int32_t *max_out)
+{
+    if (max_out) *max_out = 0;
+    if (n1 <= 0 || n2 <= 0) return 0;
+    int32_t *prev = (int32_t *)calloc((size_t)n1 + 1, sizeof(int32_t));
+    int32_t *cur = (int32_t *)calloc((size_t)n1 + 1, sizeof(int32_t));
+    if (!prev || !cur) { free(prev); free(cur); return -1; }
+    int32_t best = 0;
+    for (int64_t y = 1; y <= n2; ++y) {
+        const int32_t by = b[y - 1];
+        cur[0] = 0;
+        for (int64_t x = 1; x <= n1; ++x) {
+            int32_t s = (a[x - 1] == by) ? 3 : -3;
+            int32_t h = imax(cur[x - 1] - 2,
+                        imax(prev[x] - 2, imax(prev[x - 1] + s, 0)));
+            cur[x] = h;
+            best = imax(best, h);
+        }
+        int32_t *t = prev; prev = cur; cur = t;
+    }
+    int32_t r = prev[n1];
+    free(prev); free(cur);
+    if (max_out) *max_out = best;
+    return r;
+}
+
+/* Row-0 boundary of EditDistance/ED.cpp:48-54 (textbook=0) or of the GPU
+ * reference's GPU_setInit (textbook=1). */
+static inline int32_t ed_top(int64_t x, int textbook)
+{
+    if (x == 0) return 0;
+    return textbook ? (int32_t)x : (int32_t)(x - 1);
+}
+
+/* EditDistance/ED.cpp:45-81 with rolling rows. */
+int32_t oracle_ed(const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
+                  int textbook)
+{
+    if (n1 < 0 || n2 < 0) return -1;
+    if (n2 == 0) return ed_top(n1, textbook);
+    if (n1 == 0) return (int32_t)n2;
+    int32_t *prev = (int32_t *)malloc(((size_t)n1 + 1) * sizeof(int32_t));
+    int32_t *cur = (int32_t *)malloc(((size_t)n1 + 1) * sizeof(int32_t));
+    if (!prev || !cur) { free(prev); free(cur); return -1; }
+    for (int64_t x = 0; x <= n1; ++x) prev[x] = ed_top(x, textbook);
+    for (int64_t y = 1; y <= n2; ++y) {
+        const int32_t by = b[y - 1];
+        cur[0] = (int32_t)y;
+        for (int64_t x = 1; x <= n1; ++x) {
+            int32_t d = imin(prev[x] + 1, imin(cur[x - 1] + 1, prev[x - 1] + 1));
+            if (a[x - 1] == by) d = prev[x - 1];
+            cur[x] = d;
+        }
+        int32_t *t = prev; prev = cur; cur = t;
+    }
+    int32_t r = prev[n1];
+    free(prev); free(cur);
+    return r;
+}
+
+/* Full-table forms: table has (n2+1)*(n1+1) cells, row-major with stride
+ * n1+1, filled in the reference's loop order.  Returns the bottom-right
+ * cell.  The caller owns `table`. */
+int32_t oracle_table(int problem, const int32_t *a, int64_t n1,
+                     const int32_t *b, int64_t n2, int textbook,
+                     int32_t *table)
+{
+    const int64_t st = n1 + 1;
+    for (int64_t x = 0; x <= n1; ++x)
+        table[x] = (problem == ORACLE_ED) ? ed_top(x, textbook) : 0;
+    for (int64_t y = 0; y <= n2; ++y)
+        table[y * st] = (problem == ORACLE_ED) ? (int32_t)y : 0;
+    for (int64_t y = 1; y <= n2; ++y) {
+        for (int64_t x = 1; x <= n1; ++x) {
+            const int64_t i = y * st + x;
+            const int32_t up = table[i - st], left = table[i - 1],
+                          diag = table[i - st - 1];
+            const int eq = a[x - 1] == b[y - 1];
+            int32_t v;
+            if (problem == ORACLE_LCS) {
+                v = eq ? diag + 1 : imax(left, up);
+            } else if (problem == ORACLE_SW) {
+                v = imax(left - 2, imax(up - 2, imax(diag + (eq ? 3 : -3), 0)));
+            } else {
+                v = imin(up + 1, imin(left + 1, diag + 1));
+                if (eq) v = diag;
+            }
+            table[i] = v;
+        }
+    }
+    return table[n2 * st + n1];
+}
+
+/* Last row H(n2, 0..n1) written to out[0..n1]; used to check boundary rows
+ * exchanged between strips.  Returns 0 on success. */
+int oracle_last_row(int problem, const int32_t *a, int64_t n1,
+                    const int32_t *b, int64_t n2, int textbook, int32_t *out)
+{
+    int32_t *prev = (int32_t *)malloc(((size_t)n1 + 1) * sizeof(int32_t));
+    int32_t *cur = (int32_t *)malloc(((size_t)n1 + 1) * sizeof(int32_t));
+    if (!prev || !cur) { free(prev); free(cur); return -1; }
+    for (int64_t x = 0; x <= n1; ++x)
+        prev[x] = (problem == ORACLE_ED) ? ed_top(x, textbook) : 0;
+    for (int64_t y = 1; y <= n2; ++y) {
+        const int32_t by = b[y - 1];
+        cur[0] = (problem == ORACLE_ED) ? (int32_t)y : 0;
+        for (int64_t x = 1; x <= n1; ++x) {
+            const int32_t up = prev[x], left = cur[x - 1], diag = prev[x - 1];
+            const int eq = a[x - 1] == by;
+            int32_t v;
+            if (problem == ORACLE_LCS) {
+                v = eq ? diag + 1 : imax(left, up);
+            } else if (problem == ORACLE_SW) {
+                v = imax(left - 2, imax(up - 2, imax(diag + (eq ? 3 : -3), 0)));
+            } else {
+                v = imin(up + 1, imin(left + 1, diag + 1));
+                if (eq) v = diag;
+            }
+            cur[x] = v;
+        }
+        int32_t *t = prev; prev = cur; cur = t;
+    }
+    memcpy(out, prev, ((size_t)n1 + 1) * sizeof(int32_t));
+    free(prev); free(cur);
+    return 0;
+}
+
+/* Batch helper: results[i] = last score of pair i (ED uses ED.cpp's
+ * boundary), maxs[i] = SW maximum (0 for LCS/ED). */
+void oracle_batch(const int32_t *problems, const int32_t *const *as,
+                  const int64_t *n1s, const int32_t *const *bs,
+                  const int64_t *n2s, int64_t npairs, int32_t *results,
+                  int32_t *maxs)
+{
+    for (int64_t i = 0; i < npairs; ++i) {
+        int32_t m = 0;
+        switch (problems[i]) {
+        case ORACLE_LCS: results[i] = oracle_lcs(as[i], n1s[i], bs[i], n2s[i]); break;
+        case ORACLE_SW: results[i] = oracle_sw(as[i], n1s[i], bs[i], n2s[i], &m); break;
+        default: results[i] = oracle_ed(as[i], n1s[i], bs[i], n2s[i], 0); break;
+        }
+        if (maxs) maxs[i] = m;
+    }
+}

@@ -1,0 +1,136 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle on the
+same seeded inputs.  Integer work, so the bar is bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpumemefficient_amd import Job, Tile
+
+pytestmark = pytest.mark.gpu
+
+# (rows_per_lane, waves, chunk) of every compiled kernel variant.
+VARIANTS = [
+    (1, 4, 16), (2, 4, 16), (4, 4, 16), (1, 4, 32), (2, 4, 32), (4, 4, 32),
+    (1, 4, 64), (2, 4, 64), (4, 4, 64), (1, 1, 32), (2, 1, 32), (4, 1, 32), (8, 1, 32),
+    (1, 8, 16), (2, 8, 16), (1, 2, 32), (2, 2, 32), (4, 2, 32),
+]
+
+
+def tile_of(R, W, K, **kw):
+    return Tile(tile_w=K, strip_rows=64 * W * R, rows_per_lane=R, **kw)
+
+
+def rand_pair(rng, n1, n2, alpha):
+    return rng.integers(0, alpha, n1, dtype=np.int32), rng.integers(0, alpha, n2, dtype=np.int32)
+
+
+def check(ctx, prob, a, b, tile=None, textbook=False):
+    r = ctx.align(prob, a, b, tile=tile, ed_textbook=textbook)
+    last, mx = O.score(prob, a, b, textbook=textbook)
+    print(f"  {prob} n1={a.size} n2={b.size} tile={tile} -> {r.last}/{r.max_score} oracle {last}/{mx}", flush=True)
+    assert r.status == 0
+    assert r.last == last
+    if prob == "SW":
+        assert r.max_score == mx
+
+
+@pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
+def test_small_square_default(ctx, prob):
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 3, 7, 63, 64, 65, 100, 257, 1000):
+        a, b = rand_pair(rng, n, n, 4)
+        check(ctx, prob, a, b)
+
+
+@pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
+def test_non_square(ctx, prob):
+    rng = np.random.default_rng(12)
+    for n1, n2 in [(1, 300), (300, 1), (5, 700), (700, 5), (129, 1000), (1000, 129), (3000, 777), (64, 4096)]:
+        a, b = rand_pair(rng, n1, n2, 4)
+        check(ctx, prob, a, b)
+
+
+@pytest.mark.parametrize("R,W,K", VARIANTS)
+@pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
+def test_every_variant(ctx, prob, R, W, K):
+    rng = np.random.default_rng(100 + R * 31 + W * 7 + K)
+    # Several strips per pair (inter-workgroup flags) and ragged edges.
+    for n1, n2 in [(777, 64 * W * R * 3 + 17), (100, 64 * W * R + 1), (1500, 1333)]:
+        a, b = rand_pair(rng, n1, n2, 4)
+        check(ctx, prob, a, b, tile=tile_of(R, W, K))
+
+
+@pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
+@pytest.mark.parametrize("alpha", [2, 8, 9, 20, 32, 33, 200])
+def test_alphabets(ctx, prob, alpha):
+    rng = np.random.default_rng(alpha)
+    a, b = rand_pair(rng, 900, 1100, alpha)
+    check(ctx, prob, a, b)
+
+
+@pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
+def test_arbitrary_symbols(ctx, prob):
+    rng = np.random.default_rng(5)
+    syms = np.array([-7, 2**31 - 1, -2**31, 12345, 0, 99], dtype=np.int32)
+    a = syms[rng.integers(0, syms.size, 800)]
+    b = syms[rng.integers(0, syms.size, 600)]
+    check(ctx, prob, a, b)
+
+
+def test_ed_textbook(ctx):
+    rng = np.random.default_rng(6)
+    for n1, n2 in [(1, 1), (2, 2), (50, 70), (1000, 999)]:
+        a, b = rand_pair(rng, n1, n2, 4)
+        check(ctx, "ED", a, b, textbook=True)
+    # Known answer: arr1=[1,2], arr2=[2,3] -> ED.cpp 1, textbook 2.
+    a, b = np.array([1, 2], np.int32), np.array([2, 3], np.int32)
+    assert ctx.align("ED", a, b).last == 1
+    assert ctx.align("ED", a, b, ed_textbook=True).last == 2
+
+
+@pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
+def test_empty(ctx, prob):
+    e = np.zeros(0, np.int32)
+    x = np.arange(5, dtype=np.int32)
+    for a, b in [(e, e), (e, x), (x, e)]:
+        check(ctx, prob, a, b)
+
+
+def test_identity_closed_forms(ctx):
+    rng = np.random.default_rng(7)
+    a = rng.integers(0, 4, 5000, dtype=np.int32)
+    assert ctx.align("LCS", a, a).last == 5000
+    assert ctx.align("ED", a, a).last == 0
+    r = ctx.align("SW", a, a)
+    assert r.last == 3 * 5000 and r.max_score == 3 * 5000
+
+
+def test_batch_mixed(ctx):
+    rng = np.random.default_rng(8)
+    jobs = []
+    for i in range(200):
+        prob = ["LCS", "ED", "SW"][i % 3]
+        n1, n2 = int(rng.integers(0, 1500)), int(rng.integers(0, 1500))
+        a, b = rand_pair(rng, n1, n2, 4 if prob != "ED" else 20)
+        jobs.append(Job(prob, a, b))
+    res = ctx.align_batch(jobs)
+    for j, r in zip(jobs, res):
+        last, mx = O.score(j.problem, j.a, j.b)
+        assert r.status == 0
+        assert (r.last, r.max_score if j.problem == "SW" else 0) == (last, mx)
+
+
+def test_plan_persistent_equals_per_pair(ctx):
+    rng = np.random.default_rng(9)
+    jobs = [Job("SW", *rand_pair(rng, int(rng.integers(64, 2000)), int(rng.integers(64, 2000)), 4))
+            for _ in range(40)]
+    plan = ctx.plan(jobs)
+    plan.run()
+    r1 = plan.results()
+    plan.run(per_pair=True)
+    r2 = plan.results()
+    for j, x, y in zip(jobs, r1, r2):
+        last, mx = O.sw(j.a, j.b)
+        assert (x.last, x.max_score) == (last, mx)
+        assert (y.last, y.max_score) == (last, mx)
+    plan.close()
