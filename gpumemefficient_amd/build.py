@@ -102,8 +102,8 @@ def build_drivers() -> None:
         d = BIN / pdir
         d.mkdir(parents=True, exist_ok=True)
         _run([CXX, "-std=c++17", "-O2", f"-DWF_DRIVER_PROBLEM_{prob}", f"-DWF_DRIVER_STYLE_{style.upper()}",
-              f"-DWF_DRIVER_POLICY_{policy.upper()}", f"-I{INCLUDE}", str(src), "-o", str(d / name),
-              f"-L{LIB}", "-lwfdp", f"-Wl,-rpath,$ORIGIN/../../lib"])
+              f"-I{INCLUDE}", str(src), "-o", str(d / name),
+              f"-L{LIB}", f"-lwfdp_compat_{policy}", "-lwfdp", "-Wl,-rpath,$ORIGIN/../../lib"])
     gen = CSRC / "wf_generator.cpp"
     if gen.exists():
         for pdir in ("LCS", "SmithWaterman", "EditDistance", "LCS-SHFL"):

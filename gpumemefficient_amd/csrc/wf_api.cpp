@@ -34,7 +34,8 @@ namespace {
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
-    int ensure(size_t bytes)
+    // zero_fill: clear new memory (boundary rows must not hold a stale tag).
+    int ensure(size_t bytes, bool zero_fill = false)
     {
         if (bytes <= cap) return WF_OK;
         if (ptr) {
@@ -48,6 +49,11 @@ struct DevBuf {
             (void)hipGetLastError();
             ptr = nullptr;
             return WF_ERR_NOMEM;
+        }
+        if (zero_fill && hipMemset(ptr, 0, want) != hipSuccess) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            return WF_ERR_DEVICE;
         }
         cap = want;
         return WF_OK;
@@ -63,14 +69,13 @@ struct DevBuf {
 }  // namespace
 
 struct WfArenas {
-    DevBuf codes, pairs, tasks, bnd, flags, res, counters;
+    DevBuf codes, pairs, tasks, bnd, res, counters;
     void release()
     {
         codes.release();
         pairs.release();
         tasks.release();
         bnd.release();
-        flags.release();
         res.release();
         counters.release();
     }
@@ -82,7 +87,8 @@ struct wf_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipDeviceProp_t prop{};
     double timeout_s = 10.0;
-    WfArenas scratch;  // used by wf_align / wf_align_batch
+    uint32_t epoch = 0;  // launches so far; tags strip-boundary values
+    WfArenas scratch;    // used by wf_align / wf_align_batch
 };
 
 struct wf_plan {
@@ -270,7 +276,7 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     const int64_t strip = 64LL * v->waves * v->rows_per_lane;
 
     // Codes: a then b per pair, 16-byte aligned.
-    int64_t code_bytes = 0, bnd_ints = 0, nflags = 0;
+    int64_t code_bytes = 0, bnd_ints = 0;
     for (int64_t i = 0; i < njobs; ++i) {
         const wf_job &j = jobs[i];
         WfPairDev &p = ph->pairs[i];
@@ -290,8 +296,6 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         p.ngroups = (int32_t)((j.n2 + strip - 1) / strip);
         p.bnd_off = bnd_ints;
         bnd_ints += (int64_t)(p.ngroups - 1) * j.n1;
-        p.flag_off = (int32_t)nflags;
-        nflags += p.ngroups;
     }
     ph->codes.assign((size_t)std::max<int64_t>(code_bytes, 16), 0);
     for (int64_t i = 0; i < njobs; ++i) {
@@ -323,7 +327,6 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         ph->task_end[i] = (int64_t)ph->tasks.size();
     }
     ph->bnd_ints = bnd_ints;
-    ph->nflags = nflags;
     ph->order = order;
     return WF_OK;
 }
@@ -335,8 +338,7 @@ static int upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
     if ((st = ar->codes.ensure(ph->codes.size())) != WF_OK) return st;
     if ((st = ar->pairs.ensure(std::max<size_t>(1, ph->pairs.size()) * sizeof(WfPairDev))) != WF_OK) return st;
     if ((st = ar->tasks.ensure(std::max<size_t>(1, ntasks) * sizeof(WfTaskDev))) != WF_OK) return st;
-    if ((st = ar->bnd.ensure(std::max<int64_t>(1, ph->bnd_ints) * sizeof(int32_t))) != WF_OK) return st;
-    if ((st = ar->flags.ensure(std::max<int64_t>(1, ph->nflags) * sizeof(int32_t))) != WF_OK) return st;
+    if ((st = ar->bnd.ensure(std::max<int64_t>(1, ph->bnd_ints) * sizeof(uint64_t), true)) != WF_OK) return st;
     if ((st = ar->res.ensure(std::max<int64_t>(1, ph->npairs) * sizeof(WfResultDev))) != WF_OK) return st;
     if ((st = ar->counters.ensure((size_t)(ph->npairs + 1) * sizeof(int32_t))) != WF_OK) return st;
     if (hipMemcpyAsync(ar->codes.ptr, ph->codes.data(), ph->codes.size(), hipMemcpyHostToDevice,
@@ -375,15 +377,20 @@ static int run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double 
     const WfVariant *v = ph->variant;
     const int64_t ntasks = (int64_t)ph->tasks.size();
     const int64_t ncounters = ph->npairs + 1;
-    if (wf_launch_reset((int32_t *)ar->flags.ptr, ph->nflags, (WfResultDev *)ar->res.ptr, ph->npairs,
-                        (int32_t *)ar->counters.ptr, ncounters, ctx->stream) != hipSuccess)
+    if (wf_launch_reset((WfResultDev *)ar->res.ptr, ph->npairs, (int32_t *)ar->counters.ptr, ncounters,
+                        ctx->stream) != hipSuccess)
         return WF_ERR_DEVICE;
+    if (++ctx->epoch == 0) {
+        // 2^32 launches: every tag value has been used; clear the arena.
+        if (hipMemsetAsync(ar->bnd.ptr, 0, ar->bnd.cap, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+        ctx->epoch = 1;
+    }
     WfKernelArgs ka{};
     ka.pairs = (const WfPairDev *)ar->pairs.ptr;
     ka.tasks = (const WfTaskDev *)ar->tasks.ptr;
     ka.codes = (const uint8_t *)ar->codes.ptr;
-    ka.bnd = (int32_t *)ar->bnd.ptr;
-    ka.flags = (int32_t *)ar->flags.ptr;
+    ka.bnd = (uint64_t *)ar->bnd.ptr;
+    ka.epoch = ctx->epoch;
     ka.res = (WfResultDev *)ar->res.ptr;
     ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
     const int maxg = max_resident_groups(ctx, v);

@@ -18,11 +18,11 @@ enum {
 struct WfPairDev {
     int64_t a_off;     // byte offset of a's codes
     int64_t b_off;     // byte offset of b's codes
-    int64_t bnd_off;   // int offset of this pair's strip-boundary rows
+    int64_t bnd_off;   // element offset of this pair's strip-boundary rows
     int32_t n1, n2;
     int32_t prob;      // WFK_LCS / WFK_ED / WFK_SW
     int32_t mode;      // WFK_MODE_* bits
-    int32_t flag_off;  // offset of this pair's per-strip progress counters
+    int32_t pad;
     int32_t ngroups;   // strips of (waves * 64 * rows_per_lane) rows
 };
 
@@ -46,13 +46,14 @@ struct WfKernelArgs {
     const WfPairDev *pairs;
     const WfTaskDev *tasks;
     const uint8_t *codes;
-    int32_t *bnd;            // strip-boundary rows
-    int32_t *flags;          // per-strip published-column counters
+    uint64_t *bnd;           // strip-boundary rows: (epoch << 32) | value per column
     WfResultDev *res;
     int32_t *counter;        // task claim counter for this launch
     int64_t timeout_ticks;   // s_memrealtime ticks (100 MHz) a dependency wait may take
     int32_t task_begin;
     int32_t task_end;
+    uint32_t epoch;          // launch number, tags the boundary values of this launch
+    int32_t pad;
 };
 
 #endif

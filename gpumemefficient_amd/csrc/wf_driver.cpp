@@ -1,0 +1,143 @@
+// wf_driver.cpp -- drop-in drivers with the reference's argv and stdout.
+//
+// One source, compiled per binary (gpumemefficient_amd/build.py):
+//   -DWF_DRIVER_PROBLEM_{LCS,SW,ED}
+//   -DWF_DRIVER_STYLE_LCS2    LCS/GPU_Hyperlane_{Share,Cache}.cpp:49-137
+//                             argv "e1 e2" (itr = 100); also accepts
+//                             "e1 e2 tX tY itr" like the SW/ED Share drivers
+//   -DWF_DRIVER_STYLE_SHARE5  SmithWaterman/GPU_Hyperlane_Share.cpp:48-141,
+//                             EditDistance/GPU_Hyperlane_Share.cpp:48-144
+//                             argv "e1 e2 tX tY itr"
+//   -DWF_DRIVER_STYLE_CACHE2  SmithWaterman/GPU_Hyperlane_Cache.cpp,
+//                             EditDistance/GPU_Hyperlane_Cache.cpp: "e1 e2", itr = 100
+//   -DWF_DRIVER_STYLE_SHFL    LCS-SHFL/GPU_Hyperlane_{Share,Cache}.cpp: "e1 e2",
+//                             prints the total seconds of 100 calls
+// The input is ./Data/str1_2_<e1>_str2_2_<e2>.txt; e1 and e2 only name the
+// file, the sizes come from its header (LCS/GPU_Hyperlane_Share.cpp:14-34,
+// 65-79).  The timed region is the loop of host calls, as in the reference.
+// The reference also allocates an unused (n2+3)*(n1+1) host table (68 GB at
+// 2^17); these drivers pass a null table instead.
+#include <sys/time.h>
+
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "wfdp_compat.h"
+
+using namespace std;
+
+static void readInputData(const string &path, int &n1, int &n2, vector<int> &arr1, vector<int> &arr2)
+{
+    ifstream in(path.c_str());
+    if (!in) {
+        cout << "ERROR: Input file cannot be opened!" << endl;
+        exit(EXIT_FAILURE);
+    }
+    in >> n1 >> n2;
+    arr1.resize(n1 > 0 ? n1 : 0);
+    arr2.resize(n2 > 0 ? n2 : 0);
+    for (int i = 0; i < n1; i++) in >> arr1[i];
+    for (int i = 0; i < n2; i++) in >> arr2[i];
+}
+
+static double elapsed_ms(const timeval &a, const timeval &b)
+{
+    return (double)(b.tv_sec - a.tv_sec) * 1000.0 + (double)(b.tv_usec - a.tv_usec) / 1000.0;
+}
+
+int main(int argc, char **argv)
+{
+    int nn1 = 0, nn2 = 0, tX = 0, tY = 0, itr = 100;
+    bool tiles = false;
+#if defined(WF_DRIVER_STYLE_SHARE5)
+    if (argc != 6) {
+#if defined(WF_DRIVER_PROBLEM_SW)
+        cout << "Incorrect Input Parameters. Must be two string sizes ,two tile sizes and one total iteration step."
+             << endl;
+#else
+        cout << "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one total iteration step."
+             << endl;
+#endif
+        exit(EXIT_FAILURE);
+    }
+    nn1 = atoi(argv[1]);
+    nn2 = atoi(argv[2]);
+    tX = atoi(argv[3]);
+    tY = atoi(argv[4]);
+    itr = atoi(argv[5]);
+    tiles = true;
+#if defined(WF_DRIVER_PROBLEM_SW)
+    cout << nn1 << " " << nn2 << " " << tX << " " << tY << endl;
+#endif
+#elif defined(WF_DRIVER_STYLE_LCS2)
+    if (argc == 6) {
+        nn1 = atoi(argv[1]);
+        nn2 = atoi(argv[2]);
+        tX = atoi(argv[3]);
+        tY = atoi(argv[4]);
+        itr = atoi(argv[5]);
+        tiles = true;
+    } else if (argc == 3) {
+        nn1 = atoi(argv[1]);
+        nn2 = atoi(argv[2]);
+    } else {
+        cout << "Incorrect Input Parameters. Must be two string sizes." << endl;
+        exit(EXIT_FAILURE);
+    }
+#else
+    if (argc != 3) {
+        cout << "Incorrect Input Parameters. Must be two string sizes." << endl;
+        exit(EXIT_FAILURE);
+    }
+    nn1 = atoi(argv[1]);
+    nn2 = atoi(argv[2]);
+#endif
+    if (itr < 1) itr = 1;
+
+    ostringstream path;
+    path << "./Data/str1_2_" << nn1 << "_str2_2_" << nn2 << ".txt";
+
+    int n1 = 0, n2 = 0;
+    vector<int> arr1, arr2;
+    const int paddX = 3, paddY = 1;
+    readInputData(path.str(), n1, n2, arr1, arr2);
+    cout << "input data loaded" << endl;
+
+    int score = 0;
+    timeval tbegin, tend;
+    gettimeofday(&tbegin, NULL);
+    for (int i = 0; i < itr; i++) {
+#if defined(WF_DRIVER_PROBLEM_LCS)
+#if defined(WF_DRIVER_STYLE_SHFL)
+        score = LCS(n1, n2, arr1.data(), arr2.data());
+#else
+        score = tiles ? LCS(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr, tX, tY)
+                      : LCS(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr);
+#endif
+#elif defined(WF_DRIVER_PROBLEM_SW)
+        score = tiles ? SW(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr, tX, tY)
+                      : SW(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr);
+#else
+        score = tiles ? ED(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr, tX, tY)
+                      : ED(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr);
+#endif
+    }
+    gettimeofday(&tend, NULL);
+    const double ms = elapsed_ms(tbegin, tend);
+
+#if defined(WF_DRIVER_PROBLEM_LCS)
+    cout << "lcs length: " << score << endl;
+#else
+    cout << "last: " << score << endl;
+#endif
+#if defined(WF_DRIVER_STYLE_SHFL)
+    cout << "execution time: " << ms / 1000.0 << " second." << endl;
+#else
+    cout << "execution time: " << ms / itr << " ms." << endl;
+#endif
+    return 0;
+}

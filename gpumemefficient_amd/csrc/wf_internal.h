@@ -21,7 +21,6 @@ struct WfPlanHost {
     std::vector<uint8_t> host_only;
     std::vector<uint64_t> cells;
     int64_t bnd_ints = 0;
-    int64_t nflags = 0;
     int grid = 0;
 };
 

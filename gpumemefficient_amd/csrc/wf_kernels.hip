@@ -17,9 +17,10 @@
 // one s_barrier; wave w runs D chunks behind wave w-1, which is exactly the
 // time lane 63 of wave w-1 needs to produce the columns lane 0 of wave w
 // reads.  The strip's bottom row (lane 63 of the last wave) is the only data
-// that goes to HBM: it is copied out every chunk and published with a
-// release store of the number of finished columns; the next strip's first
-// wave polls that counter with an acquire load.  This replaces the
+// that goes to HBM: after every chunk its finished columns are stored as
+// 64-bit (launch epoch, value) words with relaxed device-scope stores, and
+// the next strip's first wave prefetches them one chunk ahead and re-polls
+// any whose tag is not yet this launch's epoch.  This replaces the
 // reference's one-launch-per-strip + volatile flag scheme
 // (LCS/GPU_Hyperlane_Share.cu:102-121, 741-767) with one persistent launch
 // that claims strips from an atomic counter in dependency order.
@@ -210,22 +211,6 @@ __device__ __forceinline__ int64_t now_ticks()
     return (int64_t)__builtin_amdgcn_s_memrealtime();
 }
 
-// Wave-uniform wait until *flag >= need.  Returns false on timeout.
-__device__ bool wait_flag(const int32_t *flag, int need, int64_t timeout)
-{
-    int v = __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-    if (v >= need) return true;
-    const int64_t t0 = now_ticks();
-    for (;;) {
-        __builtin_amdgcn_s_sleep(2);
-        v = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-        if (v >= need) return true;
-        if (now_ticks() - t0 > timeout) return false;
-    }
-}
-
 // One strip (`g`) of one pair, swept by the whole workgroup.
 template <int P, bool CMP, int R, int W, int K, int AW>
 __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
@@ -255,22 +240,38 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
     const int total = nch + (W - 1) * D;
     const bool has_prev = g > 0;
     const bool has_next = g + 1 < pd.ngroups;
-    const int32_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
-    int32_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
-    const int32_t *flag_prev = ka.flags + pd.flag_off + (g - 1);
-    int32_t *flag_cur = ka.flags + pd.flag_off + g;
+    // Strip-boundary rows hold (launch epoch << 32 | value): a reader knows
+    // a column is ready when its tag equals this launch's epoch, so the rows
+    // need no separate progress flag, no release/acquire cache maintenance
+    // and no clearing between launches.
+    const uint64_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
+    uint64_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
+    const uint64_t tag = (uint64_t)ka.epoch << 32;
     // Row 0 of the table in this kernel's value representation.
     const int topv = (P == WFK_ED && !(pd.mode & WFK_MODE_TEXTBOOK)) ? -1 : 0;
 
-    // Fill ring[0] and the code window with columns [c*K, c*K + K).
-    auto fill = [&](int c, uint32_t code_pre) -> bool {
+    // Top-boundary loads for columns [c*K, c*K+K) (wave 0, lanes < K).
+    auto load_top = [&](int c) -> uint64_t {
         const int x = c * K + lane;
-        if (has_prev) {
-            const int need = min(n1, (c + 1) * K);
-            if (!wait_flag(flag_prev, need, ka.timeout_ticks)) return false;
+        if (!has_prev || lane >= K || x >= n1) return tag;  // nothing to wait for
+        return __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // Make sure every lane's value carries this epoch (re-polling the missing
+    // ones), then store columns [c*K, c*K+K) into ring[0] and the code window.
+    auto fill = [&](int c, uint64_t top, uint32_t code_pre) -> bool {
+        const int x = c * K + lane;
+        if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)ka.epoch) != 0) {
+            const int64_t t0 = now_ticks();
+            for (;;) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((top >> 32) != (uint64_t)ka.epoch)
+                    top = __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)ka.epoch) == 0) break;
+                if (now_ticks() - t0 > ka.timeout_ticks) return false;
+            }
         }
         if (lane < K && x < n1) {
-            const int v = has_prev ? bprev[x] : topv;
+            const int v = has_prev ? (int)(uint32_t)top : topv;
             sh.ring[0][(x + 64) & (RS - 1)] = v;
             const int slot = x & (AW - 1);
             sh.aw[slot] = (uint8_t)code_pre;
@@ -281,36 +282,36 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
 
     if (wave == 0) {
         const uint32_t c0 = (lane < K && lane < n1) ? acodes[lane] : 0u;
-        if (!fill(0, c0) && lane == 0) sh.abort_at = -1;
+        if (!fill(0, load_top(0), c0) && lane == 0) sh.abort_at = -1;
     }
     __syncthreads();
 
-    int copied = 0;  // wave W-1: columns already copied to bnext
-    auto copy_out = [&](int upto) {
+    int copied = 0;  // wave W-1: columns already stored to bnext
+    // Store finished bottom-row columns [copied, upto) from ring[W].
+    auto copy_out = [&](int upto, bool from_ring) {
         while (copied < upto) {
             const int x = copied + lane;
-            if (x < upto) bnext[x] = sh.ring[W][(x + 64) & (RS - 1)];
+            if (x < upto) {
+                const int v = from_ring ? sh.ring[W][(x + 64) & (RS - 1)] : 0;
+                __hip_atomic_store(bnext + x, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             copied = min(upto, copied + 64);
         }
-        if (lane == 0) __hip_atomic_store(flag_cur, upto, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     };
 
     bool aborted = sh.abort_at <= -1;
     for (int j = 0; j < total && !aborted; ++j) {
         const int cj = j - wave * D;
-        // Wave 0 prefetches the next chunk's column codes from HBM/L2.
+        // Wave 0 prefetches the next chunk's column codes and top boundary.
         uint32_t code_pre = 0;
+        uint64_t top_pre = tag;
         if (wave == 0 && j + 1 < nch) {
             const int x = (j + 1) * K + lane;
             code_pre = (lane < K && x < n1) ? acodes[x] : 0u;
+            top_pre = load_top(j + 1);
         }
         if (cj >= 0 && cj < nch) {
             const int s0 = cj * K;
-            if (wave == W - 1 && has_next) {
-                // Steps < s0 are done, so lane 63 has finished columns < s0 - 63.
-                const int upto = min(n1, s0 - 63);
-                if (upto > copied) copy_out(upto);
-            }
             const bool steady = (s0 >= 63) && (s0 + K - 1 <= n1 - 1);
             if (steady)
                 run_chunk<P, CMP, R, K, AW, false, TRACK>(st, s0, lane, n1, sh.ring[wave],
@@ -318,9 +319,13 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
             else
                 run_chunk<P, CMP, R, K, AW, true, TRACK>(st, s0, lane, n1, sh.ring[wave],
                                                          sh.ring[wave + 1], sh.dummy[wave], sh.aw);
+            if (wave == W - 1 && has_next) {
+                // Steps < s0 + K are done: lane 63 has finished columns < s0 + K - 63.
+                copy_out(min(n1, s0 + K - 63), true);
+            }
         }
         if (wave == 0 && j + 1 < nch) {
-            if (!fill(j + 1, code_pre) && lane == 0) sh.abort_at = j;
+            if (!fill(j + 1, top_pre, code_pre) && lane == 0) sh.abort_at = j;
         }
         __syncthreads();
         aborted = sh.abort_at <= j;
@@ -328,15 +333,13 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
 
     if (aborted) {
         // A dependency wait timed out.  Mark the pair and release the strip
-        // below so that it does not wait out its own timeout in turn.
-        if (threadIdx.x == 0) {
-            atomicOr(&ka.res[pair].status, 1);
-            if (has_next)
-                __hip_atomic_store(flag_cur, 0x7fffffff, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        // below (tagged placeholder values) so that it does not wait out its
+        // own timeout in turn.
+        if (threadIdx.x == 0) atomicOr(&ka.res[pair].status, 1);
+        if (wave == W - 1 && has_next) copy_out(n1, false);
         return;
     }
-    if (wave == W - 1 && has_next) copy_out(n1);
+    if (wave == W - 1 && has_next) copy_out(n1, true);
 
     // Bottom-right cell.
     const int64_t ylast = (int64_t)n2 - 1;
@@ -396,12 +399,10 @@ __global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
 
 // Zero progress counters, results and claim counters before a launch (a
 // kernel rather than hipMemsetAsync so the sequence can be graph-captured).
-__global__ void wf_reset(int32_t *flags, int64_t nflags, WfResultDev *res, int64_t nres,
-                         int32_t *counters, int64_t ncounters)
+__global__ void wf_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters)
 {
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = i0; i < nflags; i += stride) flags[i] = 0;
     for (int64_t i = i0; i < nres; i += stride) res[i] = WfResultDev{0, 0, 0, 0};
     for (int64_t i = i0; i < ncounters; i += stride) counters[i] = 0;
 }
@@ -426,17 +427,15 @@ extern "C" const WfVariant wf_variants[] = {
 };
 extern "C" const int wf_num_variants = sizeof(wf_variants) / sizeof(wf_variants[0]);
 
-extern "C" hipError_t wf_launch_reset(int32_t *flags, int64_t nflags, WfResultDev *res, int64_t nres,
-                                      int32_t *counters, int64_t ncounters, hipStream_t stream)
+extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
+                                      hipStream_t stream)
 {
-    int64_t n = nflags;
-    if (nres > n) n = nres;
+    int64_t n = nres;
     if (ncounters > n) n = ncounters;
     int blocks = (int)((n + 255) / 256);
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(wfk::wf_reset, dim3(blocks), dim3(256), 0, stream, flags, nflags, res, nres,
-                       counters, ncounters);
+    hipLaunchKernelGGL(wfk::wf_reset, dim3(blocks), dim3(256), 0, stream, res, nres, counters, ncounters);
     return hipGetLastError();
 }
 

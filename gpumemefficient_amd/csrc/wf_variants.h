@@ -17,8 +17,8 @@ struct WfVariant {
 extern "C" const WfVariant wf_variants[];
 extern "C" const int wf_num_variants;
 
-extern "C" hipError_t wf_launch_reset(int32_t *flags, int64_t nflags, WfResultDev *res, int64_t nres,
-                                      int32_t *counters, int64_t ncounters, hipStream_t stream);
+extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
+                                      hipStream_t stream);
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
 

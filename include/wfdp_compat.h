@@ -1,0 +1,38 @@
+/*
+ * wfdp_compat.h -- the reference's per-problem host functions with their
+ * exact C++ signatures, implemented over wf_align() (include/wfdp.h).
+ *
+ * Link libwfdp_compat_share.so or libwfdp_compat_cache.so (plus libwfdp.so)
+ * in place of the reference's .cu objects and its driver .cpp files build
+ * unchanged.  The two libraries differ only in the tile policy of the calls
+ * that carry no tile arguments:
+ *   share: LCS 32x256 (LCS/GPU_Hyperlane_Share.cu:703-704)
+ *   cache: 256x1024, strip boundaries through L2 only
+ *          (LCS/GPU_Hyperlane_Cache.cu:254-255 and the SW/ED Cache hosts)
+ *
+ * Semantics kept: return value = bottom-right DP cell; `table`, `paddX` and
+ * `paddY` are accepted and ignored (the reference writes the host table only
+ * in DEBUG builds); on a device error the process prints the error and exits
+ * with EXIT_FAILURE, as checkGPUError does (LCS/GPU_Hyperlane_Share.cu
+ * :689-694).  The device is 0, as in the reference (cudaSetDevice(0)).
+ */
+#ifndef WFDP_COMPAT_H
+#define WFDP_COMPAT_H
+
+/* LCS/GPU_Hyperlane_Share.h:5, LCS/GPU_Hyperlane_Cache.h:5 */
+int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
+/* LCS-SHFL/GPU_Hyperlane_Share.h:5, LCS-SHFL/GPU_Hyperlane_Cache.h:5 */
+int LCS(int n1, int n2, int *arr1, int *arr2);
+/* SmithWaterman/GPU_Hyperlane_Share.h:5 */
+int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
+/* SmithWaterman/GPU_Hyperlane_Cache.h:5 */
+int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
+/* EditDistance/GPU_Hyperlane_Share.h:5 */
+int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
+/* EditDistance/GPU_Hyperlane_Cache.h:5 */
+int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
+
+/* Extension used by the drop-in LCS driver's optional 5-argument form. */
+int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
+
+#endif /* WFDP_COMPAT_H */
