@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the wavefront DP hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ed|sw|lcs|batch]
+
+Default workload (the configuration the headline metric is quoted on that
+fits one GPU): edit distance (ED.cpp semantics) between two i.i.d. uniform
+20-letter sequences of 65536 symbols, seed 2 -- 4.29e9 DP cells.  A step is
+one pass of the DP over the pair with its codes resident in HBM (one
+wf_plan_run: reset + one persistent launch, synchronous).
+
+One JSON line on stdout:
+  value        GCUPS over the K timed steps (wall clock, whole job)
+  roofline     the engine kernel against the int32 VALU peak of the chip
+               (256 CUs x 64 lanes x max engine clock): achieved = algorithmic
+               int ops per launch (ED 7, SW 8, LCS 4 per cell) / mean launch
+               time from HIP events on the library's stream
+  cpu_baseline the oracle (C restatement of ED.cpp, one thread) timed on this
+               host on the same pair
+  extra        the other headline measurements: Smith-Waterman 131072^2
+               single pair, and the 4096-pair batch against the loop of one
+               launch per pair
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+OPS_PER_CELL = {"LCS": 4, "ED": 7, "SW": 8}
+PROFILES = ROOT / "profiles"
+
+
+def peak_int32_tops(info: dict) -> float:
+    """int32 VALU ops/s of the chip: one 64-lane op per CU per cycle."""
+    return info["compute_units"] * 64 * info["max_clock_mhz"] * 1e6 / 1e12
+
+
+def traffic_for(workload: str):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), as committed in profiles/."""
+    f = PROFILES / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        return d.get(workload, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def make_workload(name: str):
+    from gpumemefficient_amd import Job
+    from gpumemefficient_amd import datagen as G
+    if name == "ed":
+        a, b = G.ed_pair()
+        return [Job("ED", a, b)], "ED", {"workload": "edit_distance_65536x65536_alpha20_seed2", "n1": 65536,
+                                         "n2": 65536, "alphabet": 20, "seed": 2, "semantics": "EditDistance/ED.cpp"}
+    if name == "sw":
+        a, b = G.sw_pair()
+        return [Job("SW", a, b)], "SW", {"workload": "smith_waterman_131072x131072_alpha4_seed3_planted4096",
+                                         "n1": 131072, "n2": 131072, "scoring": "+3/-3/-2 (SmithWaterman/SW.cpp)"}
+    if name == "lcs":
+        a, b = G.lcs_pair()
+        return [Job("LCS", a, b)], "LCS", {"workload": "lcs_8192x8192_alpha4_seed1", "n1": 8192, "n2": 8192}
+    if name == "batch":
+        pairs = G.batch_pairs()
+        return [Job("SW", a, b) for a, b in pairs], "SW", {
+            "workload": "smith_waterman_batch_4096_pairs_loguniform_64_4096_seed4", "pairs": 4096}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def time_plan(plan, steps: int, warmup: int, per_pair: bool = False):
+    for _ in range(warmup):
+        plan.run(per_pair=per_pair)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kms.append(plan.run(per_pair=per_pair))   # synchronous: returns after the stream drains
+    dt = time.perf_counter() - t0
+    return dt, kms
+
+
+def cpu_baseline(jobs, prob: str, budget_s: float = 20.0):
+    """Time the oracle (single thread) on the pair; if the pair would take
+    longer than the budget, on its first rows (a band of the same table)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    a, b = jobs[0].a, jobs[0].b
+    # Probe the rate on a small band, then size the sample to the budget.
+    probe_rows = max(1, min(b.size, 256))
+    t0 = time.perf_counter()
+    O.score(prob, a, b[:probe_rows])
+    rate = a.size * probe_rows / max(1e-9, time.perf_counter() - t0)
+    rows = int(min(b.size, max(probe_rows, budget_s * rate / a.size)))
+    t0 = time.perf_counter()
+    O.score(prob, a, b[:rows])
+    dt = time.perf_counter() - t0
+    cells = a.size * rows
+    sample = (f"{prob} oracle (C restatement, -O3, 1 thread) on rows 0..{rows - 1} of the {b.size}x{a.size} "
+              f"pair ({cells:.3e} cells, {dt:.1f} s)")
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port", "sample": sample}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ed", choices=["ed", "sw", "lcs", "batch"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the SW single-pair and batch measurements")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(one huge pair would shard as a point-to-point boundary-row stream, see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    from gpumemefficient_amd import Context
+    ctx = Context(0, timeout_s=30.0)
+    info = ctx.info()
+    peak = peak_int32_tops(info)
+
+    jobs, prob, config = make_workload(args.workload)
+    cells = sum(int(j.a.size) * int(j.b.size) for j in jobs)
+    plan = ctx.plan(jobs)
+    dt, kms = time_plan(plan, args.steps, args.warmup)
+    res = plan.results()
+    desc = plan.describe()
+    kernel_ms = sum(kms) / len(kms)
+    gcups = cells * args.steps / dt / 1e9
+    achieved = OPS_PER_CELL[prob] * cells / (kernel_ms * 1e-3) / 1e12
+    config.update({"engine": desc["engine"], "rows_per_lane": desc["rows_per_lane"], "waves": desc["waves"],
+                   "chunk": desc["chunk"], "strips": desc["tasks"], "grid": desc["grid"]})
+    line = {
+        "metric": "GCUPS & % int-VALU peak; batched vs one-pair-per-launch; p50/p99 latency",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded PCG64, gpumemefficient_amd.datagen)",
+        "config": config,
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": round(peak, 3),
+            "unit": "TOP/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": traffic_for(args.workload),
+            "ops_per_cell": OPS_PER_CELL[prob],
+            "kernel_ms": round(kernel_ms, 4),
+        },
+        "result": {"last": res[0].last, "max_score": res[0].max_score} if len(res) == 1 else
+                  {"pairs": len(res), "sum_last": int(sum(r.last for r in res))},
+    }
+    plan.close()
+
+    if not args.no_extra:
+        extra = {}
+        if args.workload != "sw":
+            sj, sp, sc = make_workload("sw")
+            p = ctx.plan(sj)
+            sdt, skms = time_plan(p, 3, 1)
+            sr = p.results()[0]
+            sk = sum(skms) / len(skms)
+            scells = int(sj[0].a.size) * int(sj[0].b.size)
+            extra["sw_131072"] = {
+                "gcups": round(scells / sk / 1e6, 1), "kernel_ms": round(sk, 3),
+                "int_valu_peak_frac_8ops": round(8 * scells / (sk * 1e-3) / 1e12 / peak, 4),
+                "last": sr.last, "max_score": sr.max_score, "engine": p.describe()}
+            p.close()
+        if args.workload != "batch":
+            bj, bp, bc = make_workload("batch")
+            p = ctx.plan(bj)
+            bdt, bkms = time_plan(p, 5, 1)
+            pdt, pkms = time_plan(p, 3, 1, per_pair=True)
+            bcells = sum(int(j.a.size) * int(j.b.size) for j in bj)
+            bk, pk = sum(bkms) / len(bkms), sum(pkms) / len(pkms)
+            extra["batch_4096"] = {
+                "cells": bcells, "persistent_gcups": round(bcells / bk / 1e6, 1), "persistent_ms": round(bk, 3),
+                "per_pair_launch_gcups": round(bcells / pk / 1e6, 1), "per_pair_launch_ms": round(pk, 3),
+                "speedup": round(pk / bk, 2), "engine": p.describe()}
+            p.close()
+        line["extra"] = extra
+
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(jobs, prob)
+    ctx.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

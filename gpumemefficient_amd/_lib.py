@@ -26,6 +26,7 @@ WF_ERR_AGAIN = 7
 
 WF_TILE_ED_TEXTBOOK = 0x1
 WF_TILE_CACHE = 0x2
+WF_TILE_LOCKSTEP = 0x4
 
 WF_RUN_PERSISTENT = 0
 WF_RUN_PER_PAIR = 1
@@ -35,6 +36,7 @@ EXPORTS = [
     "wf_ctx_create", "wf_ctx_destroy", "wf_device_query", "wf_strerror", "wf_abi_version",
     "wf_ctx_set_timeout", "wf_align", "wf_align_batch", "wf_plan_create", "wf_plan_run",
     "wf_plan_results", "wf_plan_kernel_ms", "wf_plan_describe", "wf_plan_destroy",
+    "wf_variant_count", "wf_variant_info",
 ]
 
 
@@ -104,7 +106,11 @@ def load() -> C.CDLL:
     lib.wf_plan_kernel_ms.argtypes = [P]
     lib.wf_plan_kernel_ms.restype = C.c_double
     lib.wf_plan_describe.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                     C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    lib.wf_variant_count.argtypes = []
+    lib.wf_variant_count.restype = C.c_int
+    lib.wf_variant_info.argtypes = [C.c_int] + [C.POINTER(C.c_int32)] * 4
+    lib.wf_variant_info.restype = C.c_int
     lib.wf_plan_describe.restype = C.c_int
     lib.wf_plan_destroy.argtypes = [P]
     lib.wf_plan_destroy.restype = None

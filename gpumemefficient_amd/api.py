@@ -66,9 +66,11 @@ class Tile:
     rows_per_lane: int = 0
     ed_textbook: bool = False
     cache: bool = False
+    lockstep: bool = False
 
     def _c(self) -> _lib.wf_tile:
-        flags = (_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
+        flags = ((_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
+                 | (_lib.WF_TILE_LOCKSTEP if self.lockstep else 0))
         return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
 
 
@@ -124,10 +126,12 @@ class Plan:
         return [Result._from(out[i]) for i in range(self.n)]
 
     def describe(self) -> dict:
-        r, w, k, g = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        e, r, w, k, g = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         nt = C.c_int64()
-        check(self.ctx._lib.wf_plan_describe(self._h, C.byref(r), C.byref(w), C.byref(k), C.byref(nt), C.byref(g)))
-        return {"rows_per_lane": r.value, "waves": w.value, "chunk": k.value, "tasks": nt.value, "grid": g.value}
+        check(self.ctx._lib.wf_plan_describe(self._h, C.byref(e), C.byref(r), C.byref(w), C.byref(k), C.byref(nt),
+                                             C.byref(g)))
+        return {"engine": ("lockstep", "async")[e.value], "rows_per_lane": r.value, "waves": w.value,
+                "chunk": k.value, "tasks": nt.value, "grid": g.value}
 
     def close(self) -> None:
         if self._h:
@@ -196,6 +200,24 @@ class Context:
             pass
 
 
+def variants() -> list[dict]:
+    """Compiled kernel variants: engine, rows_per_lane, waves, chunk."""
+    lib = _lib.load()
+    out = []
+    for i in range(lib.wf_variant_count()):
+        e, r, w, k = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib.wf_variant_info(i, C.byref(e), C.byref(r), C.byref(w), C.byref(k)))
+        out.append({"engine": ("lockstep", "async")[e.value], "rows_per_lane": r.value, "waves": w.value,
+                    "chunk": k.value})
+    return out
+
+
+def tile_for(v: dict, **kw) -> Tile:
+    """The Tile that selects exactly variant `v` (see variants())."""
+    return Tile(tile_w=v["chunk"], strip_rows=64 * v["waves"] * v["rows_per_lane"],
+                rows_per_lane=v["rows_per_lane"], lockstep=v["engine"] == "lockstep", **kw)
+
+
 _default: Context | None = None
 
 
@@ -244,4 +266,5 @@ def ED(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
     return default_context().align("ED", a, b, t).last
 
 
-__all__ = ["Context", "Plan", "Job", "Tile", "Result", "WfError", "LCS", "SW", "ED", "default_context"]
+__all__ = ["Context", "Plan", "Job", "Tile", "Result", "WfError", "LCS", "SW", "ED", "default_context", "variants",
+           "tile_for"]

@@ -68,7 +68,7 @@ def _hip_obj(src: Path, obj: Path) -> None:
 def build_lib() -> Path:
     LIB.mkdir(exist_ok=True)
     OBJ.mkdir(exist_ok=True)
-    srcs = [CSRC / "wf_kernels.hip", CSRC / "wf_api.cpp", CSRC / "wf_server.cpp",
+    srcs = [CSRC / "wf_engine.hip", CSRC / "wf_kernels.hip", CSRC / "wf_api.cpp", CSRC / "wf_server.cpp",
             CSRC / "wf_tune.cpp"]
     srcs = [s for s in srcs if s.exists()]
     objs = [OBJ / (s.stem + ".o") for s in srcs]

@@ -171,28 +171,41 @@ static int32_t empty_score(int problem, int64_t n1, int64_t n2, int flags)
 // Variant choice
 // ----------------------------------------------------------------------------
 
-static const WfVariant *find_variant(int R, int W, int K)
+static int variant_count() { return wf_num_variants_async + wf_num_variants_lockstep; }
+
+static const WfVariant *variant_at(int i)
 {
-    for (int i = 0; i < wf_num_variants; ++i)
-        if (wf_variants[i].rows_per_lane == R && wf_variants[i].waves == W && wf_variants[i].chunk == K)
-            return &wf_variants[i];
+    if (i < 0 || i >= variant_count()) return nullptr;
+    return i < wf_num_variants_async ? &wf_variants_async[i] : &wf_variants_lockstep[i - wf_num_variants_async];
+}
+
+static const WfVariant *find_variant(int E, int R, int W, int K)
+{
+    for (int i = 0; i < variant_count(); ++i) {
+        const WfVariant *v = variant_at(i);
+        if (v->engine == E && v->rows_per_lane == R && v->waves == W && v->chunk == K) return v;
+    }
     return nullptr;
 }
 
 // Nearest compiled variant to a requested strip height / chunk width.
+// strip_rows = 64 * waves * rows_per_lane; tile_w maps to the chunk (the
+// lock-step barrier interval, or the async progress interval).
 const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t total_cells,
                                  int npairs)
 {
-    int R = 0, W = 0, K = 0;
+    int E = WF_ENGINE_ASYNC, R = 0, W = 0, K = 0;
     if (const char *env = std::getenv("WFDP_VARIANT")) {
-        if (std::sscanf(env, "%d,%d,%d", &R, &W, &K) == 3) {
-            if (const WfVariant *v = find_variant(R, W, K)) return v;
+        if (std::sscanf(env, "%d,%d,%d,%d", &E, &R, &W, &K) == 4) {
+            if (const WfVariant *v = find_variant(E, R, W, K)) return v;
         }
+        E = WF_ENGINE_ASYNC;
         R = W = K = 0;
     }
     const bool cache = tile && (tile->flags & WF_TILE_CACHE);
+    if (tile && (tile->flags & WF_TILE_LOCKSTEP)) E = WF_ENGINE_LOCKSTEP;
     if (tile && tile->rows_per_lane > 0) R = tile->rows_per_lane;
-    if (tile && tile->tile_w > 0) K = tile->tile_w <= 16 ? 16 : (tile->tile_w <= 32 ? 32 : 64);
+    if (tile && tile->tile_w > 0) K = tile->tile_w;
     if (tile && tile->strip_rows > 0) {
         const int rows = tile->strip_rows;
         if (cache) W = 1;
@@ -216,14 +229,15 @@ const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t 
     }
     if (W == 0) W = cache ? 1 : wf_default_waves(max_rows, total_cells, npairs);
     if (R == 0) R = wf_default_rows_per_lane(W, max_rows, total_cells, npairs);
-    if (K == 0) K = (W == 1) ? 32 : (W >= 8 ? 16 : 16);
-    // Snap to the nearest compiled variant.
+    if (K == 0) K = 16;
+    // Snap to the nearest compiled variant of the requested engine.
     const WfVariant *best = nullptr;
     int best_d = 1 << 30;
-    for (int i = 0; i < wf_num_variants; ++i) {
-        const WfVariant &v = wf_variants[i];
+    for (int i = 0; i < variant_count(); ++i) {
+        const WfVariant &v = *variant_at(i);
+        if (v.engine != E) continue;
         const int d = std::abs(v.waves - W) * 1000 + std::abs(v.rows_per_lane - R) * 10 +
-                      std::abs(v.chunk - K) / 16;
+                      std::abs(v.chunk - K) / 8;
         if (d < best_d) {
             best_d = d;
             best = &v;
@@ -359,7 +373,7 @@ static int upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
 static int max_resident_groups(wf_ctx *ctx, const WfVariant *v)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v->fn, 64 * v->waves, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v->fn, v->block, 0) != hipSuccess ||
         per_cu < 1) {
         (void)hipGetLastError();
         per_cu = 1;
@@ -461,6 +475,19 @@ static int download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double ke
 extern "C" {
 
 int wf_abi_version(void) { return WFDP_ABI_VERSION; }
+
+int wf_variant_count(void) { return variant_count(); }
+
+int wf_variant_info(int index, int32_t *engine, int32_t *rows_per_lane, int32_t *waves, int32_t *chunk)
+{
+    const WfVariant *v = variant_at(index);
+    if (!v) return WF_ERR_INVALID;
+    if (engine) *engine = v->engine;
+    if (rows_per_lane) *rows_per_lane = v->rows_per_lane;
+    if (waves) *waves = v->waves;
+    if (chunk) *chunk = v->chunk;
+    return WF_OK;
+}
 
 const char *wf_strerror(int status)
 {
@@ -595,11 +622,12 @@ int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n)
 
 double wf_plan_kernel_ms(const wf_plan *plan) { return plan ? plan->kernel_ms : -1.0; }
 
-int wf_plan_describe(const wf_plan *plan, int32_t *rows_per_lane, int32_t *waves, int32_t *chunk,
-                     int64_t *ntasks, int32_t *grid)
+int wf_plan_describe(const wf_plan *plan, int32_t *engine, int32_t *rows_per_lane, int32_t *waves,
+                     int32_t *chunk, int64_t *ntasks, int32_t *grid)
 {
     if (!plan) return WF_ERR_INVALID;
     const WfVariant *v = plan->host.variant;
+    if (engine) *engine = v->engine;
     if (rows_per_lane) *rows_per_lane = v->rows_per_lane;
     if (waves) *waves = v->waves;
     if (chunk) *chunk = v->chunk;

@@ -1,0 +1,125 @@
+// wf_cells.h -- device building blocks shared by the wavefront engines:
+// cell functions (the three reference recurrences), per-lane state and the
+// one-column step of a lane.  Included by wf_kernels.hip and wf_engine.hip.
+#ifndef WF_CELLS_H
+#define WF_CELLS_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wf_device.h"
+
+namespace wfk {
+
+constexpr int kDppWaveShr1 = 0x138;  // DPP_CTRL wave_shr:1 (GFX9 family)
+
+constexpr int pow2_at_least(int v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Chunk geometry for chunk width K (steps between barriers).
+template <int K>
+struct ChunkGeom {
+    static_assert(K == 16 || K == 32 || K == 64, "K must divide 64");
+    // Lag in chunks between consecutive waves: lane 0 of wave w reads, at
+    // chunk-aligned groups of 4 columns, values lane 63 of wave w-1 wrote
+    // 63..66 steps later in its own step count.
+    static constexpr int D = (K + 62) / K + 1;
+    // LDS ring of bottom-lane values, indexed by (column + 64) mod RS.
+    static constexpr int RS = pow2_at_least((D + 2) * K);
+};
+
+// ----------------------------------------------------------------------------
+// Cell functions.  `ca` is the column symbol code, `t` the lane's row entry.
+// BFE mode: t is a per-row table, ca a bit offset into it.
+// CMP mode: t is the row symbol code (0x100 for padding rows), ca the code.
+// ----------------------------------------------------------------------------
+template <int P, bool CMP>
+struct Cell;
+
+template <bool CMP>
+struct Cell<WFK_LCS, CMP> {
+    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
+    {
+        if (CMP) return valid ? bc : 0x100u;
+        return valid ? (1u << bc) : 0u;
+    }
+    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
+    {
+        const int eq = CMP ? (int)(ca == t) : (int)__builtin_amdgcn_ubfe(t, ca, 1);
+        return max(max(up, left), diag + eq);
+    }
+};
+
+template <bool CMP>
+struct Cell<WFK_ED, CMP> {
+    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
+    {
+        if (CMP) return valid ? bc : 0x100u;
+        return valid ? ~(1u << bc) : 0xffffffffu;
+    }
+    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
+    {
+        const int ne = CMP ? (int)(ca != t) : (int)__builtin_amdgcn_ubfe(t, ca, 1);
+        return min(min(up, left), diag + ne - 2);
+    }
+};
+
+template <bool CMP>
+struct Cell<WFK_SW, CMP> {
+    // BFE: 4-bit signed field at bit 4*code holds s+2: 5 on a match, -1 else.
+    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
+    {
+        if (CMP) return valid ? bc : 0x100u;
+        return valid ? ((~(0xfu << bc)) | (5u << bc)) : 0xffffffffu;
+    }
+    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
+    {
+        const int s2 = CMP ? ((ca == t) ? 5 : -1) : __builtin_amdgcn_sbfe((int)t, ca, 4);
+        const int m = max(max(up, left), diag + s2);
+        // max(m - 2, 0) for m >= 0: unsigned saturating subtract.
+        return (int)__builtin_elementwise_sub_sat((uint32_t)m, 2u);
+    }
+};
+
+template <int R>
+struct LaneState {
+    int h[R];        // H(row r, last column) -- the next step's "left"
+    int dg;          // H(row above row 0, last column) -- the next step's "diag"
+    int best;        // SW running maximum
+    uint32_t t[R];   // per-row table / code
+};
+
+template <int P, bool CMP, int R, bool MASKED, bool TRACK>
+__device__ __forceinline__ void step(LaneState<R> &st, int inj, uint32_t ca, bool valid)
+{
+    const int up0 = __builtin_amdgcn_update_dpp(inj, st.h[R - 1], kDppWaveShr1, 0xf, 0xf, false);
+    int up = up0;
+    int diag = st.dg;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int left = st.h[r];
+        int h = Cell<P, CMP>::f(up, left, diag, ca, st.t[r]);
+        if (MASKED) {
+            h = valid ? h : left;
+            if (TRACK) st.best = max(st.best, valid ? h : 0);
+        } else if (TRACK) {
+            st.best = max(st.best, h);
+        }
+        diag = left;
+        st.h[r] = h;
+        up = h;
+    }
+    st.dg = MASKED ? (valid ? up0 : st.dg) : up0;
+}
+
+__device__ __forceinline__ int64_t now_ticks()
+{
+    return (int64_t)__builtin_amdgcn_s_memrealtime();
+}
+
+}  // namespace wfk
+
+#endif

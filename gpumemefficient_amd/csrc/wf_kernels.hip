@@ -39,112 +39,9 @@
 
 #include "wf_device.h"
 
+#include "wf_cells.h"
+
 namespace wfk {
-
-constexpr int kDppWaveShr1 = 0x138;  // DPP_CTRL wave_shr:1 (GFX9 family)
-
-constexpr int pow2_at_least(int v)
-{
-    int p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
-// Chunk geometry for chunk width K (steps between barriers).
-template <int K>
-struct ChunkGeom {
-    static_assert(K == 16 || K == 32 || K == 64, "K must divide 64");
-    // Lag in chunks between consecutive waves: lane 0 of wave w reads, at
-    // chunk-aligned groups of 4 columns, values lane 63 of wave w-1 wrote
-    // 63..66 steps later in its own step count.
-    static constexpr int D = (K + 62) / K + 1;
-    // LDS ring of bottom-lane values, indexed by (column + 64) mod RS.
-    static constexpr int RS = pow2_at_least((D + 2) * K);
-};
-
-// ----------------------------------------------------------------------------
-// Cell functions.  `ca` is the column symbol code, `t` the lane's row entry.
-// BFE mode: t is a per-row table, ca a bit offset into it.
-// CMP mode: t is the row symbol code (0x100 for padding rows), ca the code.
-// ----------------------------------------------------------------------------
-template <int P, bool CMP>
-struct Cell;
-
-template <bool CMP>
-struct Cell<WFK_LCS, CMP> {
-    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
-    {
-        if (CMP) return valid ? bc : 0x100u;
-        return valid ? (1u << bc) : 0u;
-    }
-    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
-    {
-        const int eq = CMP ? (int)(ca == t) : (int)__builtin_amdgcn_ubfe(t, ca, 1);
-        return max(max(up, left), diag + eq);
-    }
-};
-
-template <bool CMP>
-struct Cell<WFK_ED, CMP> {
-    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
-    {
-        if (CMP) return valid ? bc : 0x100u;
-        return valid ? ~(1u << bc) : 0xffffffffu;
-    }
-    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
-    {
-        const int ne = CMP ? (int)(ca != t) : (int)__builtin_amdgcn_ubfe(t, ca, 1);
-        return min(min(up, left), diag + ne - 2);
-    }
-};
-
-template <bool CMP>
-struct Cell<WFK_SW, CMP> {
-    // BFE: 4-bit signed field at bit 4*code holds s+2: 5 on a match, -1 else.
-    static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
-    {
-        if (CMP) return valid ? bc : 0x100u;
-        return valid ? ((~(0xfu << bc)) | (5u << bc)) : 0xffffffffu;
-    }
-    static __device__ __forceinline__ int f(int up, int left, int diag, uint32_t ca, uint32_t t)
-    {
-        const int s2 = CMP ? ((ca == t) ? 5 : -1) : __builtin_amdgcn_sbfe((int)t, ca, 4);
-        const int m = max(max(up, left), diag + s2);
-        // max(m - 2, 0) for m >= 0: unsigned saturating subtract.
-        return (int)__builtin_elementwise_sub_sat((uint32_t)m, 2u);
-    }
-};
-
-template <int R>
-struct LaneState {
-    int h[R];        // H(row r, last column) -- the next step's "left"
-    int dg;          // H(row above row 0, last column) -- the next step's "diag"
-    int best;        // SW running maximum
-    uint32_t t[R];   // per-row table / code
-};
-
-template <int P, bool CMP, int R, bool MASKED, bool TRACK>
-__device__ __forceinline__ void step(LaneState<R> &st, int inj, uint32_t ca, bool valid)
-{
-    const int up0 = __builtin_amdgcn_update_dpp(inj, st.h[R - 1], kDppWaveShr1, 0xf, 0xf, false);
-    int up = up0;
-    int diag = st.dg;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int left = st.h[r];
-        int h = Cell<P, CMP>::f(up, left, diag, ca, st.t[r]);
-        if (MASKED) {
-            h = valid ? h : left;
-            if (TRACK) st.best = max(st.best, valid ? h : 0);
-        } else if (TRACK) {
-            st.best = max(st.best, h);
-        }
-        diag = left;
-        st.h[r] = h;
-        up = h;
-    }
-    st.dg = MASKED ? (valid ? up0 : st.dg) : up0;
-}
 
 // LDS workspace of one workgroup.
 template <int W, int K, int AW>
@@ -204,11 +101,6 @@ __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, in
                 cap[j] = st.h[R - 1];
         }
     }
-}
-
-__device__ __forceinline__ int64_t now_ticks()
-{
-    return (int64_t)__builtin_amdgcn_s_memrealtime();
 }
 
 // One strip (`g`) of one pair, swept by the whole workgroup.
@@ -415,9 +307,9 @@ __global__ void wf_reset(WfResultDev *res, int64_t nres, int32_t *counters, int6
 #include "wf_variants.h"
 
 #define WF_VARIANT(R_, W_, K_) \
-    {R_, W_, K_, reinterpret_cast<const void *>(&wfk::wf_engine<R_, W_, K_>)}
+    {WF_ENGINE_LOCKSTEP, R_, W_, K_, 64 * (W_), reinterpret_cast<const void *>(&wfk::wf_engine<R_, W_, K_>)}
 
-extern "C" const WfVariant wf_variants[] = {
+extern "C" const WfVariant wf_variants_lockstep[] = {
     WF_VARIANT(1, 4, 16), WF_VARIANT(2, 4, 16), WF_VARIANT(4, 4, 16),
     WF_VARIANT(1, 4, 32), WF_VARIANT(2, 4, 32), WF_VARIANT(4, 4, 32),
     WF_VARIANT(1, 4, 64), WF_VARIANT(2, 4, 64), WF_VARIANT(4, 4, 64),
@@ -425,7 +317,7 @@ extern "C" const WfVariant wf_variants[] = {
     WF_VARIANT(1, 8, 16), WF_VARIANT(2, 8, 16),
     WF_VARIANT(1, 2, 32), WF_VARIANT(2, 2, 32), WF_VARIANT(4, 2, 32),
 };
-extern "C" const int wf_num_variants = sizeof(wf_variants) / sizeof(wf_variants[0]);
+extern "C" const int wf_num_variants_lockstep = sizeof(wf_variants_lockstep) / sizeof(wf_variants_lockstep[0]);
 
 extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
                                       hipStream_t stream)
@@ -443,5 +335,5 @@ extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKer
                                        hipStream_t stream)
 {
     void *args[] = {const_cast<WfKernelArgs *>(ka)};
-    return hipLaunchKernel(v->fn, dim3(grid), dim3(64 * v->waves), args, 0, stream);
+    return hipLaunchKernel(v->fn, dim3(grid), dim3(v->block), args, 0, stream);
 }

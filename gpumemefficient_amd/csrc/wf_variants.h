@@ -7,15 +7,21 @@
 
 #include "wf_device.h"
 
+enum { WF_ENGINE_LOCKSTEP = 0, WF_ENGINE_ASYNC = 1 };
+
 struct WfVariant {
+    int engine;         // WF_ENGINE_*
     int rows_per_lane;  // R
-    int waves;          // W: wave64s per workgroup (strip height = 64*W*R rows)
-    int chunk;          // K: steps between workgroup barriers / boundary publishes
-    const void *fn;     // __global__ wf_engine<R, W, K>
+    int waves;          // W: compute wave64s per workgroup (strip height = 64*W*R rows)
+    int chunk;          // lock-step: steps between barriers; async: steps between progress updates
+    int block;          // threads per workgroup (async adds one I/O wave)
+    const void *fn;     // the __global__ kernel
 };
 
-extern "C" const WfVariant wf_variants[];
-extern "C" const int wf_num_variants;
+extern "C" const WfVariant wf_variants_lockstep[];
+extern "C" const int wf_num_variants_lockstep;
+extern "C" const WfVariant wf_variants_async[];
+extern "C" const int wf_num_variants_async;
 
 extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
                                       hipStream_t stream);

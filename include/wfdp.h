@@ -71,6 +71,8 @@ typedef struct {
 #define WF_TILE_ED_TEXTBOOK 0x1 /* ED row 0 = x (reference GPU_setInit), not ED.cpp's x-1 */
 #define WF_TILE_CACHE 0x2       /* hyperlane-Cache policy: one wave per workgroup, all
                                    strip boundaries through L2 (GPU_Hyperlane_Cache.cu) */
+#define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine instead of the default
+                                   asynchronous one (kept for comparison) */
 
 typedef struct {
     int32_t last;      /* bottom-right cell: the score the reference prints */
@@ -129,8 +131,13 @@ int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile
 int wf_plan_run(wf_plan *plan, int mode);                 /* enqueue + wait */
 int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n);
 double wf_plan_kernel_ms(const wf_plan *plan);            /* last run, HIP events */
-int wf_plan_describe(const wf_plan *plan, int32_t *rows_per_lane, int32_t *waves,
-                     int32_t *chunk, int64_t *ntasks, int32_t *grid);
+int wf_plan_describe(const wf_plan *plan, int32_t *engine, int32_t *rows_per_lane,
+                     int32_t *waves, int32_t *chunk, int64_t *ntasks, int32_t *grid);
+
+/* Compiled kernel variants (engine 0 = lock-step, 1 = asynchronous). */
+int wf_variant_count(void);
+int wf_variant_info(int index, int32_t *engine, int32_t *rows_per_lane, int32_t *waves,
+                    int32_t *chunk);
 void wf_plan_destroy(wf_plan *plan);
 
 #ifdef __cplusplus

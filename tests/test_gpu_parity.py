@@ -4,20 +4,11 @@ import numpy as np
 import pytest
 
 import oracle as O
-from gpumemefficient_amd import Job, Tile
+from gpumemefficient_amd import Job, Tile, tile_for, variants
 
 pytestmark = pytest.mark.gpu
 
-# (rows_per_lane, waves, chunk) of every compiled kernel variant.
-VARIANTS = [
-    (1, 4, 16), (2, 4, 16), (4, 4, 16), (1, 4, 32), (2, 4, 32), (4, 4, 32),
-    (1, 4, 64), (2, 4, 64), (4, 4, 64), (1, 1, 32), (2, 1, 32), (4, 1, 32), (8, 1, 32),
-    (1, 8, 16), (2, 8, 16), (1, 2, 32), (2, 2, 32), (4, 2, 32),
-]
-
-
-def tile_of(R, W, K, **kw):
-    return Tile(tile_w=K, strip_rows=64 * W * R, rows_per_lane=R, **kw)
+VARIANTS = variants()
 
 
 def rand_pair(rng, n1, n2, alpha):
@@ -50,14 +41,22 @@ def test_non_square(ctx, prob):
         check(ctx, prob, a, b)
 
 
-@pytest.mark.parametrize("R,W,K", VARIANTS)
+@pytest.mark.parametrize("v", VARIANTS, ids=lambda v: f"{v['engine']}-R{v['rows_per_lane']}W{v['waves']}K{v['chunk']}")
 @pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
-def test_every_variant(ctx, prob, R, W, K):
+def test_every_variant(ctx, prob, v):
+    R, W, K = v["rows_per_lane"], v["waves"], v["chunk"]
     rng = np.random.default_rng(100 + R * 31 + W * 7 + K)
-    # Several strips per pair (inter-workgroup flags) and ragged edges.
-    for n1, n2 in [(777, 64 * W * R * 3 + 17), (100, 64 * W * R + 1), (1500, 1333)]:
+    t = tile_for(v)
+    # Several strips per pair (strip-to-strip exchange through HBM), ragged
+    # edges, a pair narrower than one wave's lag, and one longer than the
+    # LDS rings and code window.
+    for n1, n2 in [(777, 64 * W * R * 3 + 17), (100, 64 * W * R + 1), (1500, 1333), (5000, 64 * W * R * 2),
+                   (10, 64 * W * R * 2 + 5)]:
         a, b = rand_pair(rng, n1, n2, 4)
-        check(ctx, prob, a, b, tile=tile_of(R, W, K))
+        check(ctx, prob, a, b, tile=t)
+    plan = ctx.plan([Job(prob, a, b)], t)
+    assert plan.describe()["engine"] == v["engine"] and plan.describe()["waves"] == W
+    plan.close()
 
 
 @pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
