@@ -27,6 +27,7 @@ WF_ERR_AGAIN = 7
 WF_TILE_ED_TEXTBOOK = 0x1
 WF_TILE_CACHE = 0x2
 WF_TILE_LOCKSTEP = 0x4
+WF_TILE_ASYNC = 0x8
 
 WF_RUN_PERSISTENT = 0
 WF_RUN_PER_PAIR = 1
@@ -36,7 +37,9 @@ EXPORTS = [
     "wf_ctx_create", "wf_ctx_destroy", "wf_device_query", "wf_strerror", "wf_abi_version",
     "wf_ctx_set_timeout", "wf_align", "wf_align_batch", "wf_plan_create", "wf_plan_run",
     "wf_plan_results", "wf_plan_kernel_ms", "wf_plan_describe", "wf_plan_destroy",
-    "wf_variant_count", "wf_variant_info",
+    "wf_variant_count", "wf_variant_info", "wf_plan_capture", "wf_autotune", "wf_tune_load", "wf_tune_save",
+    "wf_tune_clear", "wf_server_create", "wf_server_submit", "wf_server_poll", "wf_server_wait_idle",
+    "wf_server_stats", "wf_server_replay", "wf_server_destroy",
 ]
 
 
@@ -101,6 +104,16 @@ def load() -> C.CDLL:
     lib.wf_plan_create.restype = C.c_int
     lib.wf_plan_run.argtypes = [P, C.c_int]
     lib.wf_plan_run.restype = C.c_int
+    lib.wf_plan_capture.argtypes = [P, C.c_int]
+    lib.wf_plan_capture.restype = C.c_int
+    lib.wf_autotune.argtypes = [P, C.POINTER(wf_job), C.c_int64, C.c_int, C.POINTER(wf_tile), C.POINTER(C.c_double)]
+    lib.wf_autotune.restype = C.c_int
+    lib.wf_tune_load.argtypes = [C.c_char_p]
+    lib.wf_tune_load.restype = C.c_int
+    lib.wf_tune_save.argtypes = [C.c_char_p]
+    lib.wf_tune_save.restype = C.c_int
+    lib.wf_tune_clear.argtypes = []
+    lib.wf_tune_clear.restype = C.c_int
     lib.wf_plan_results.argtypes = [P, C.POINTER(wf_result), C.c_int64]
     lib.wf_plan_results.restype = C.c_int
     lib.wf_plan_kernel_ms.argtypes = [P]
@@ -120,8 +133,8 @@ def load() -> C.CDLL:
 
 
 def _bind_server(lib: C.CDLL) -> None:
-    """Bind the serving entry points when the library exports them."""
-    from . import _server_abi  # noqa: F401  (declares argtypes lazily)
+    """Bind the serving entry points."""
+    from . import _server_abi
     _server_abi.bind(lib)
 
 

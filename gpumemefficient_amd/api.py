@@ -66,12 +66,19 @@ class Tile:
     rows_per_lane: int = 0
     ed_textbook: bool = False
     cache: bool = False
-    lockstep: bool = False
+    engine: str = ""      # "lockstep", "async" or "" (default)
 
     def _c(self) -> _lib.wf_tile:
         flags = ((_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
-                 | (_lib.WF_TILE_LOCKSTEP if self.lockstep else 0))
+                 | (_lib.WF_TILE_LOCKSTEP if self.engine == "lockstep" else 0)
+                 | (_lib.WF_TILE_ASYNC if self.engine == "async" else 0))
         return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
+
+    @classmethod
+    def _from(cls, t: _lib.wf_tile) -> "Tile":
+        eng = "lockstep" if t.flags & _lib.WF_TILE_LOCKSTEP else ("async" if t.flags & _lib.WF_TILE_ASYNC else "")
+        return cls(t.tile_w, t.strip_rows, t.rows_per_lane, bool(t.flags & _lib.WF_TILE_ED_TEXTBOOK),
+                   bool(t.flags & _lib.WF_TILE_CACHE), eng)
 
 
 @dataclass
@@ -113,6 +120,12 @@ class Plan:
         check(self.ctx._lib.wf_plan_run(self._h, _lib.WF_RUN_PER_PAIR if per_pair else _lib.WF_RUN_PERSISTENT),
               "wf_plan_run")
         return self.kernel_ms
+
+    def capture(self, per_pair: bool = False) -> None:
+        """Capture this plan's launches into a hipGraph; later run() calls of
+        the same mode replay it."""
+        check(self.ctx._lib.wf_plan_capture(self._h, _lib.WF_RUN_PER_PAIR if per_pair else _lib.WF_RUN_PERSISTENT),
+              "wf_plan_capture")
 
     @property
     def kernel_ms(self) -> float:
@@ -182,6 +195,16 @@ class Context:
     def plan(self, jobs: Iterable[Job], tile: Tile | None = None) -> Plan:
         return Plan(self, jobs, tile)
 
+    def autotune(self, jobs: Iterable[Job], reps: int = 2) -> tuple[Tile, float]:
+        """Time every compiled variant on `jobs`; returns the fastest tile and
+        its kernel time.  Later plans of the same workload class without an
+        explicit tile use it."""
+        ja = _JobArray(jobs)
+        t = _lib.wf_tile()
+        ms = C.c_double()
+        check(self._lib.wf_autotune(self._h, ja.c, ja.n, reps, C.byref(t), C.byref(ms)), "wf_autotune")
+        return Tile._from(t), ms.value
+
     def close(self) -> None:
         if self._h:
             self._lib.wf_ctx_destroy(self._h)
@@ -215,7 +238,19 @@ def variants() -> list[dict]:
 def tile_for(v: dict, **kw) -> Tile:
     """The Tile that selects exactly variant `v` (see variants())."""
     return Tile(tile_w=v["chunk"], strip_rows=64 * v["waves"] * v["rows_per_lane"],
-                rows_per_lane=v["rows_per_lane"], lockstep=v["engine"] == "lockstep", **kw)
+                rows_per_lane=v["rows_per_lane"], engine=v["engine"], **kw)
+
+
+def tune_load(path) -> None:
+    check(_lib.load().wf_tune_load(str(path).encode()), f"wf_tune_load({path})")
+
+
+def tune_save(path) -> None:
+    check(_lib.load().wf_tune_save(str(path).encode()), f"wf_tune_save({path})")
+
+
+def tune_clear() -> None:
+    _lib.load().wf_tune_clear()
 
 
 _default: Context | None = None

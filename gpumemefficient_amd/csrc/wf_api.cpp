@@ -28,75 +28,14 @@
 #include "wf_internal.h"
 #include "wf_variants.h"
 
-namespace {
-
-// Grow-only device buffer.
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t cap = 0;
-    // zero_fill: clear new memory (boundary rows must not hold a stale tag).
-    int ensure(size_t bytes, bool zero_fill = false)
-    {
-        if (bytes <= cap) return WF_OK;
-        if (ptr) {
-            (void)hipFree(ptr);
-            ptr = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 256);
-        want = (want + 255) & ~size_t(255);
-        if (hipMalloc(&ptr, want) != hipSuccess) {
-            (void)hipGetLastError();
-            ptr = nullptr;
-            return WF_ERR_NOMEM;
-        }
-        if (zero_fill && hipMemset(ptr, 0, want) != hipSuccess) {
-            (void)hipFree(ptr);
-            ptr = nullptr;
-            return WF_ERR_DEVICE;
-        }
-        cap = want;
-        return WF_OK;
-    }
-    void release()
-    {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        cap = 0;
-    }
-};
-
-}  // namespace
-
-struct WfArenas {
-    DevBuf codes, pairs, tasks, bnd, res, counters;
-    void release()
-    {
-        codes.release();
-        pairs.release();
-        tasks.release();
-        bnd.release();
-        res.release();
-        counters.release();
-    }
-};
-
-struct wf_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipDeviceProp_t prop{};
-    double timeout_s = 10.0;
-    uint32_t epoch = 0;  // launches so far; tags strip-boundary values
-    WfArenas scratch;    // used by wf_align / wf_align_batch
-};
-
 struct wf_plan {
     wf_ctx *ctx = nullptr;
     WfArenas own;
     WfPlanHost host;
     double kernel_ms = 0.0;
     bool ran = false;
+    hipGraph_t graph[2] = {nullptr, nullptr};      // per run mode, when captured
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 // ----------------------------------------------------------------------------
@@ -188,22 +127,34 @@ static const WfVariant *find_variant(int E, int R, int W, int K)
     return nullptr;
 }
 
-// Nearest compiled variant to a requested strip height / chunk width.
+// Kernel variant for a plan, in order of precedence: explicit tile fields,
+// the WFDP_VARIANT environment override ("engine,R,W,K"), the autotuned
+// table (wf_tune.cpp) for this workload and device, built-in defaults.
 // strip_rows = 64 * waves * rows_per_lane; tile_w maps to the chunk (the
 // lock-step barrier interval, or the async progress interval).
-const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t total_cells,
-                                 int npairs)
+const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job *jobs, int64_t njobs,
+                                 int64_t max_rows, int64_t total_cells)
 {
-    int E = WF_ENGINE_ASYNC, R = 0, W = 0, K = 0;
+    int E = WF_ENGINE_LOCKSTEP, R = 0, W = 0, K = 0;
     if (const char *env = std::getenv("WFDP_VARIANT")) {
         if (std::sscanf(env, "%d,%d,%d,%d", &E, &R, &W, &K) == 4) {
             if (const WfVariant *v = find_variant(E, R, W, K)) return v;
         }
-        E = WF_ENGINE_ASYNC;
+        E = WF_ENGINE_LOCKSTEP;
         R = W = K = 0;
     }
     const bool cache = tile && (tile->flags & WF_TILE_CACHE);
+    const bool explicit_tile = tile && (tile->tile_w > 0 || tile->strip_rows > 0 || tile->rows_per_lane > 0);
+    if (!explicit_tile && !cache && ctx) {
+        int p, b, bk, te, tr, tw, tk;
+        wf_tune_key(jobs, njobs, &p, &b, &bk);
+        if (wf_tune_lookup(ctx->prop.gcnArchName, ctx->prop.multiProcessorCount, p, b, bk, &te, &tr, &tw, &tk)) {
+            if (const WfVariant *v = find_variant(te, tr, tw, tk)) return v;
+        }
+    }
+    if (tile && (tile->flags & WF_TILE_ASYNC)) E = WF_ENGINE_ASYNC;
     if (tile && (tile->flags & WF_TILE_LOCKSTEP)) E = WF_ENGINE_LOCKSTEP;
+    if (cache) E = WF_ENGINE_ASYNC;
     if (tile && tile->rows_per_lane > 0) R = tile->rows_per_lane;
     if (tile && tile->tile_w > 0) K = tile->tile_w;
     if (tile && tile->strip_rows > 0) {
@@ -227,9 +178,9 @@ const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t 
             R = std::max(1, rows / (64 * W));
         }
     }
-    if (W == 0) W = cache ? 1 : wf_default_waves(max_rows, total_cells, npairs);
-    if (R == 0) R = wf_default_rows_per_lane(W, max_rows, total_cells, npairs);
-    if (K == 0) K = 16;
+    if (W == 0) W = cache ? 1 : wf_default_waves(max_rows, total_cells, (int)njobs);
+    if (R == 0) R = wf_default_rows_per_lane(W, max_rows, total_cells, (int)njobs);
+    if (K == 0) K = (E == WF_ENGINE_LOCKSTEP) ? 64 : 32;
     // Snap to the nearest compiled variant of the requested engine.
     const WfVariant *best = nullptr;
     int best_d = 1 << 30;
@@ -246,19 +197,24 @@ const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t 
     return best;
 }
 
+// Built-in defaults (measured on MI355X: SW 2^17 single pair, ED 2^16, the
+// 4096-pair batch): strips of 512 rows = 4 waves x 2 rows per lane; smaller
+// strips for pairs too short to fill four waves.
 int wf_default_waves(int64_t max_rows, int64_t total_cells, int npairs)
 {
     (void)total_cells;
-    if (npairs == 1 && max_rows >= 256) return 4;
-    return max_rows >= 1024 ? 4 : 1;
+    (void)npairs;
+    if (max_rows >= 512) return 4;
+    if (max_rows >= 256) return 2;
+    return 1;
 }
 
 int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, int npairs)
 {
     (void)total_cells;
     (void)npairs;
-    if (waves >= 4) return max_rows >= 65536 ? 2 : 1;
-    return max_rows >= 512 ? 2 : 1;
+    if (max_rows >= 64LL * waves * 2) return 2;
+    return 1;
 }
 
 // ----------------------------------------------------------------------------
@@ -268,7 +224,6 @@ int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, i
 int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                   WfPlanHost *ph)
 {
-    (void)ctx;
     ph->npairs = njobs;
     ph->pairs.assign(njobs, WfPairDev{});
     ph->host_result.assign(njobs, 0);
@@ -284,7 +239,7 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         total_cells += j.n1 * j.n2;
         ph->cells[i] = (uint64_t)(j.n1 * j.n2);
     }
-    const WfVariant *v = wf_pick_variant(tile, max_rows, total_cells, (int)njobs);
+    const WfVariant *v = wf_pick_variant(ctx, tile, jobs, njobs, max_rows, total_cells);
     if (!v) return WF_ERR_NOKERNEL;
     ph->variant = v;
     const int64_t strip = 64LL * v->waves * v->rows_per_lane;
@@ -345,8 +300,14 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     return WF_OK;
 }
 
-static int upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
+int wf_upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
 {
+    return wf_upload_plan_async(ctx, ph, ar, ctx->stream, true);
+}
+
+int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync)
+{
+    (void)ctx;
     const size_t ntasks = ph->tasks.size();
     int st;
     if ((st = ar->codes.ensure(ph->codes.size())) != WF_OK) return st;
@@ -355,22 +316,24 @@ static int upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
     if ((st = ar->bnd.ensure(std::max<int64_t>(1, ph->bnd_ints) * sizeof(uint64_t), true)) != WF_OK) return st;
     if ((st = ar->res.ensure(std::max<int64_t>(1, ph->npairs) * sizeof(WfResultDev))) != WF_OK) return st;
     if ((st = ar->counters.ensure((size_t)(ph->npairs + 1) * sizeof(int32_t))) != WF_OK) return st;
+    // Launch epoch: zero on allocation, so it never matches a zeroed boundary word.
+    if ((st = ar->epoch.ensure(sizeof(uint32_t), true)) != WF_OK) return st;
     if (hipMemcpyAsync(ar->codes.ptr, ph->codes.data(), ph->codes.size(), hipMemcpyHostToDevice,
-                       ctx->stream) != hipSuccess)
+                       stream) != hipSuccess)
         return WF_ERR_DEVICE;
     if (!ph->pairs.empty() &&
         hipMemcpyAsync(ar->pairs.ptr, ph->pairs.data(), ph->pairs.size() * sizeof(WfPairDev),
-                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                       hipMemcpyHostToDevice, stream) != hipSuccess)
         return WF_ERR_DEVICE;
     if (ntasks &&
         hipMemcpyAsync(ar->tasks.ptr, ph->tasks.data(), ntasks * sizeof(WfTaskDev),
-                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                       hipMemcpyHostToDevice, stream) != hipSuccess)
         return WF_ERR_DEVICE;
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (sync && hipStreamSynchronize(stream) != hipSuccess) return WF_ERR_DEVICE;
     return WF_OK;
 }
 
-static int max_resident_groups(wf_ctx *ctx, const WfVariant *v)
+int wf_max_resident_groups(wf_ctx *ctx, const WfVariant *v)
 {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v->fn, v->block, 0) != hipSuccess ||
@@ -385,52 +348,62 @@ static int max_resident_groups(wf_ctx *ctx, const WfVariant *v)
     return per_cu * ctx->prop.multiProcessorCount;
 }
 
-// Enqueue reset + engine launches and time the engine part.
-static int run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *kernel_ms)
+// Enqueue the reset kernel and the engine launch(es) of a plan on the
+// context stream (also used inside hipGraph capture: no memset nodes, and
+// the epoch that tags boundary values is advanced on the device).
+int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int maxg, hipStream_t stream)
 {
     const WfVariant *v = ph->variant;
     const int64_t ntasks = (int64_t)ph->tasks.size();
     const int64_t ncounters = ph->npairs + 1;
     if (wf_launch_reset((WfResultDev *)ar->res.ptr, ph->npairs, (int32_t *)ar->counters.ptr, ncounters,
-                        ctx->stream) != hipSuccess)
+                        (uint32_t *)ar->epoch.ptr, stream) != hipSuccess)
         return WF_ERR_DEVICE;
-    if (++ctx->epoch == 0) {
-        // 2^32 launches: every tag value has been used; clear the arena.
-        if (hipMemsetAsync(ar->bnd.ptr, 0, ar->bnd.cap, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
-        ctx->epoch = 1;
-    }
     WfKernelArgs ka{};
     ka.pairs = (const WfPairDev *)ar->pairs.ptr;
     ka.tasks = (const WfTaskDev *)ar->tasks.ptr;
     ka.codes = (const uint8_t *)ar->codes.ptr;
     ka.bnd = (uint64_t *)ar->bnd.ptr;
-    ka.epoch = ctx->epoch;
     ka.res = (WfResultDev *)ar->res.ptr;
+    ka.epoch = (const uint32_t *)ar->epoch.ptr;
     ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
-    const int maxg = max_resident_groups(ctx, v);
     ph->grid = 0;
-    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
-    if (ntasks > 0) {
-        if (mode == WF_RUN_PER_PAIR) {
-            for (int64_t k = 0; k < ph->npairs; ++k) {
-                const int64_t i = ph->order[k];
-                const int64_t tb = ph->task_begin[i], te = ph->task_end[i];
-                if (te <= tb) continue;
-                ka.counter = (int32_t *)ar->counters.ptr + 1 + i;
-                ka.task_begin = (int32_t)tb;
-                ka.task_end = (int32_t)te;
-                const int grid = (int)std::min<int64_t>(te - tb, maxg);
-                ph->grid = std::max(ph->grid, grid);
-                if (wf_launch_engine(v, grid, &ka, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
-            }
-        } else {
-            ka.counter = (int32_t *)ar->counters.ptr;
-            ka.task_begin = 0;
-            ka.task_end = (int32_t)ntasks;
-            const int grid = (int)std::min<int64_t>(ntasks, maxg);
-            ph->grid = grid;
-            if (wf_launch_engine(v, grid, &ka, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (ntasks == 0) return WF_OK;
+    if (mode == WF_RUN_PER_PAIR) {
+        for (int64_t k = 0; k < ph->npairs; ++k) {
+            const int64_t i = ph->order[k];
+            const int64_t tb = ph->task_begin[i], te = ph->task_end[i];
+            if (te <= tb) continue;
+            ka.counter = (int32_t *)ar->counters.ptr + 1 + i;
+            ka.task_begin = (int32_t)tb;
+            ka.task_end = (int32_t)te;
+            const int grid = (int)std::min<int64_t>(te - tb, maxg);
+            ph->grid = std::max(ph->grid, grid);
+            if (wf_launch_engine(v, grid, &ka, stream) != hipSuccess) return WF_ERR_DEVICE;
         }
+    } else {
+        ka.counter = (int32_t *)ar->counters.ptr;
+        ka.task_begin = 0;
+        ka.task_end = (int32_t)ntasks;
+        const int grid = (int)std::min<int64_t>(ntasks, maxg);
+        ph->grid = grid;
+        if (wf_launch_engine(v, grid, &ka, stream) != hipSuccess) return WF_ERR_DEVICE;
+    }
+    return WF_OK;
+}
+
+// Run a plan (directly, or by replaying its captured graph) and time the
+// device work with HIP events on the context stream.
+static int run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *kernel_ms,
+                    hipGraphExec_t exec = nullptr)
+{
+    const int maxg = wf_max_resident_groups(ctx, ph->variant);
+    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (exec) {
+        if (hipGraphLaunch(exec, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    } else {
+        const int st = wf_enqueue_plan(ctx, ph, ar, mode, maxg, ctx->stream);
+        if (st != WF_OK) return st;
     }
     if (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
     if (hipEventSynchronize(ctx->ev1) != hipSuccess) return WF_ERR_DEVICE;
@@ -440,7 +413,7 @@ static int run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double 
     return WF_OK;
 }
 
-static int download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms,
+int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms,
                             wf_result *out, int64_t n)
 {
     std::vector<WfResultDev> r((size_t)std::max<int64_t>(1, ph->npairs));
@@ -571,10 +544,10 @@ int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile
     WfPlanHost ph;
     int st = wf_build_plan(ctx, jobs, njobs, tile, &ph);
     if (st != WF_OK) return st;
-    if ((st = upload_plan(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
+    if ((st = wf_upload_plan(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
     double ms = 0.0;
     if ((st = run_plan(ctx, &ph, &ctx->scratch, WF_RUN_PERSISTENT, &ms)) != WF_OK) return st;
-    return download_results(ctx, &ph, &ctx->scratch, ms, out, njobs);
+    return wf_download_results(ctx, &ph, &ctx->scratch, ms, out, njobs);
 }
 
 int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
@@ -593,7 +566,7 @@ int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile
     wf_plan *p = new wf_plan();
     p->ctx = ctx;
     int st = wf_build_plan(ctx, jobs, njobs, tile, &p->host);
-    if (st == WF_OK) st = upload_plan(ctx, &p->host, &p->own);
+    if (st == WF_OK) st = wf_upload_plan(ctx, &p->host, &p->own);
     if (st != WF_OK) {
         p->own.release();
         delete p;
@@ -607,9 +580,35 @@ int wf_plan_run(wf_plan *plan, int mode)
 {
     if (!plan || (mode != WF_RUN_PERSISTENT && mode != WF_RUN_PER_PAIR)) return WF_ERR_INVALID;
     if (hipSetDevice(plan->ctx->device) != hipSuccess) return WF_ERR_DEVICE;
-    const int st = run_plan(plan->ctx, &plan->host, &plan->own, mode, &plan->kernel_ms);
+    const int st = run_plan(plan->ctx, &plan->host, &plan->own, mode, &plan->kernel_ms, plan->exec[mode]);
     plan->ran = (st == WF_OK);
     return st;
+}
+
+int wf_plan_capture(wf_plan *plan, int mode)
+{
+    if (!plan || (mode != WF_RUN_PERSISTENT && mode != WF_RUN_PER_PAIR)) return WF_ERR_INVALID;
+    wf_ctx *ctx = plan->ctx;
+    if (hipSetDevice(ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    if (plan->exec[mode]) return WF_OK;
+    const int maxg = wf_max_resident_groups(ctx, plan->host.variant);
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return WF_ERR_DEVICE;
+    const int st = wf_enqueue_plan(ctx, &plan->host, &plan->own, mode, maxg, ctx->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (st != WF_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return st != WF_OK ? st : WF_ERR_DEVICE;
+    }
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return WF_ERR_DEVICE;
+    }
+    plan->graph[mode] = g;
+    plan->exec[mode] = x;
+    return WF_OK;
 }
 
 int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n)
@@ -617,7 +616,7 @@ int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n)
     if (!plan || !out || n < 0) return WF_ERR_INVALID;
     if (!plan->ran) return WF_ERR_INVALID;
     if (hipSetDevice(plan->ctx->device) != hipSuccess) return WF_ERR_DEVICE;
-    return download_results(plan->ctx, &plan->host, &plan->own, plan->kernel_ms, out, n);
+    return wf_download_results(plan->ctx, &plan->host, &plan->own, plan->kernel_ms, out, n);
 }
 
 double wf_plan_kernel_ms(const wf_plan *plan) { return plan ? plan->kernel_ms : -1.0; }
@@ -641,6 +640,10 @@ void wf_plan_destroy(wf_plan *plan)
     if (!plan) return;
     (void)hipSetDevice(plan->ctx->device);
     (void)hipStreamSynchronize(plan->ctx->stream);
+    for (int m = 0; m < 2; ++m) {
+        if (plan->exec[m]) (void)hipGraphExecDestroy(plan->exec[m]);
+        if (plan->graph[m]) (void)hipGraphDestroy(plan->graph[m]);
+    }
     plan->own.release();
     delete plan;
 }

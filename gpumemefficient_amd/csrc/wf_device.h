@@ -52,8 +52,8 @@ struct WfKernelArgs {
     int64_t timeout_ticks;   // s_memrealtime ticks (100 MHz) a dependency wait may take
     int32_t task_begin;
     int32_t task_end;
-    uint32_t epoch;          // launch number, tags the boundary values of this launch
-    int32_t pad;
+    const uint32_t *epoch;   // launch number (device memory, bumped by the reset kernel before
+                             // every launch, so graph replays get fresh tags)
 };
 
 #endif

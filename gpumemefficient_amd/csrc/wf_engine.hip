@@ -61,6 +61,32 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Ordering of LDS progress counters.  LDS instructions of one wave are
+// executed in issue order, so a counter stored after a wave's ring stores
+// can only be seen once those stores are visible, and ring loads issued
+// after a counter load returned see them; only the compiler has to be kept
+// from moving the accesses across the counter (a signal fence).  Building
+// with -DWFK_LDS_HW_FENCE=1 uses s_waitcnt-based workgroup fences instead.
+#ifndef WFK_LDS_HW_FENCE
+#define WFK_LDS_HW_FENCE 1
+#endif
+__device__ __forceinline__ void lds_release()
+{
+#if WFK_LDS_HW_FENCE
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+#else
+    __atomic_signal_fence(__ATOMIC_RELEASE);
+#endif
+}
+__device__ __forceinline__ void lds_acquire()
+{
+#if WFK_LDS_HW_FENCE
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#else
+    __atomic_signal_fence(__ATOMIC_ACQUIRE);
+#endif
+}
+
 // One group of G steps starting at s0 (a multiple of G) for one compute wave.
 template <int P, bool CMP, int R, int G, int AW, int RS, bool MASKED, bool TRACK>
 __device__ __forceinline__ void run_steps(LaneState<R> &st, int s0, int lane, int n1, const int *ring_prev,
@@ -97,14 +123,14 @@ __device__ __forceinline__ void run_steps(LaneState<R> &st, int s0, int lane, in
 
 template <int P, bool CMP, int R, int W, int G, int AW, int RS>
 __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
-                            AsyncShared<W, G, AW, RS> &sh, int lane, int wave)
+                            AsyncShared<W, G, AW, RS> &sh, int lane, int wave, uint32_t epoch)
 {
     constexpr bool TRACK = (P == WFK_SW);
     const int n1 = pd.n1, n2 = pd.n2;
     const int nsteps = n1 + 63;
     const bool has_prev = g > 0;
     const bool has_next = g + 1 < pd.ngroups;
-    const uint64_t tag = (uint64_t)ka.epoch << 32;
+    const uint64_t tag = (uint64_t)epoch << 32;
 
     if (wave < W) {
         // ------------------------------------------------------ compute wave
@@ -205,6 +231,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
     uint64_t top[2];
     uint32_t code[2];
     int have = 0;       // batches loaded: columns [in_next, in_next + 64*have)
+    int staged = 0;     // leading columns of the current batch already staged
 
     auto load_batch = [&](int base, uint64_t &t, uint32_t &c) {
         const int x = base + lane;
@@ -217,17 +244,17 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
     if (!has_next) cnt_store(sh.cnt[W + 1] + lane, 1 << 30);
 
     // Idle time only counts while the compute waves are not moving either,
-    // i.e. while everything waits on the producing strip.
+    // i.e. while everything waits on the producing strip.  The wave sleeps
+    // whenever it moved no data, so it takes few issue slots from the compute
+    // wave that shares its SIMD.
     int64_t t_idle = now_ticks();
     int seen = -1;
     bool aborted = false;
     for (;;) {
         bool did = false;
         const int p0 = cnt_load(c0), pl = cnt_load(clast);
-        if (p0 + pl != seen) {
-            seen = p0 + pl;
-            did = true;
-        }
+        const bool moving = (p0 + pl != seen);
+        seen = p0 + pl;
         if (in_next < n1) {
             // Room for column in_next + 64*k + 63: its ring[0] slot must have
             // been read by wave 0 (steps < p0 done) and its code-window slot
@@ -241,32 +268,38 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                 ++have;
             }
             if (have > 0) {
-                const bool stale = (top[0] >> 32) != (uint64_t)ka.epoch;
-                if (__builtin_amdgcn_ballot_w64(stale) != 0) {
-                    // The producing strip has not stored these yet: re-poll.
-                    if (stale) top[0] = __hip_atomic_load(bprev + in_next + lane, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
-                } else {
+                const bool stale = (top[0] >> 32) != (uint64_t)epoch;
+                const uint64_t smask = __builtin_amdgcn_ballot_w64(stale);
+                // Columns [in_next, in_next + ready) of the batch are valid.
+                const int ready = smask ? (int)__builtin_ctzll(smask) : 64;
+                if (ready > staged) {
                     const int x = in_next + lane;
-                    if (x < n1) {
+                    if (lane >= staged && lane < ready && x < n1) {
                         sh.ring[0][(x + 64) & (RS - 1)] = has_prev ? (int)(uint32_t)top[0] : topv;
                         const int slot = x & (AW - 1);
                         sh.aw[slot] = (uint8_t)code[0];
                         if (slot < G) sh.aw[AW + slot] = (uint8_t)code[0];
                     }
+                    staged = ready;
+                    lds_release();
+                    cnt_store(sh.cnt[0] + lane, min(n1, in_next + ready));
+                    did = true;
+                }
+                if (staged == 64 || in_next + staged >= n1) {
                     in_next = min(n1, in_next + 64);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    cnt_store(sh.cnt[0] + lane, in_next);
+                    staged = 0;
                     top[0] = top[1];
                     code[0] = code[1];
                     --have;
-                    did = true;
+                } else if (stale) {
+                    // The producing strip has not stored these yet: re-poll.
+                    top[0] = __hip_atomic_load(bprev + in_next + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
         }
         if (has_next && out_done < n1) {
             const int avail = min(n1, pl - 63);
-            if (avail > out_done && (avail - out_done >= 16 || avail == n1)) {
+            if (avail > out_done && (avail - out_done >= 8 || avail == n1)) {
                 const int end = min(avail, out_done + 64);
                 const int x = out_done + lane;
                 if (x < end) {
@@ -274,7 +307,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                     __hip_atomic_store(bnext + x, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 out_done = end;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                lds_release();
                 cnt_store(sh.cnt[W + 1] + lane, out_done);
                 did = true;
             }
@@ -284,10 +317,9 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
             aborted = true;
             break;
         }
-        if (did) {
-            t_idle = now_ticks();
-        } else {
-            __builtin_amdgcn_s_sleep(1);
+        if (did || moving) t_idle = now_ticks();
+        if (!did) {
+            __builtin_amdgcn_s_sleep(2);
             if (now_ticks() - t_idle > ka.timeout_ticks) {
                 aborted = true;
                 if (lane == 0) __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -320,6 +352,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void wf_engine_async(WfKernelArgs ka)
     __shared__ AsyncShared<W, G, AW, RS> sh;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const uint32_t epoch = __builtin_amdgcn_readfirstlane(*ka.epoch);
     for (;;) {
         if (threadIdx.x == 0) sh.task = ka.task_begin + atomicAdd(ka.counter, 1);
         // Fresh counters for the strip.
@@ -332,12 +365,12 @@ __global__ __launch_bounds__(64 * (W + 1)) void wf_engine_async(WfKernelArgs ka)
         const WfPairDev pd = ka.pairs[td.pair];
         const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
         switch (pd.prob * 2 + (cmp ? 1 : 0)) {
-        case WFK_LCS * 2 + 0: async_strip<WFK_LCS, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_LCS * 2 + 1: async_strip<WFK_LCS, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_ED * 2 + 0: async_strip<WFK_ED, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_ED * 2 + 1: async_strip<WFK_ED, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_SW * 2 + 0: async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        default: async_strip<WFK_SW, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        case WFK_LCS * 2 + 0: async_strip<WFK_LCS, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_LCS * 2 + 1: async_strip<WFK_LCS, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_ED * 2 + 0: async_strip<WFK_ED, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_ED * 2 + 1: async_strip<WFK_ED, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_SW * 2 + 0: async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        default: async_strip<WFK_SW, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         }
         __syncthreads();
     }

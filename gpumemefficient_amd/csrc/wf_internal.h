@@ -4,11 +4,73 @@
 #define WF_INTERNAL_H
 #include <stdint.h>
 
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <vector>
 
 #include "../../include/wfdp.h"
 #include "wf_device.h"
 #include "wf_variants.h"
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    // zero_fill: clear new memory (boundary rows must not hold a stale tag).
+    int ensure(size_t bytes, bool zero_fill = false)
+    {
+        if (bytes <= cap) return WF_OK;
+        if (ptr) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        want = (want + 255) & ~size_t(255);
+        if (hipMalloc(&ptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            ptr = nullptr;
+            return WF_ERR_NOMEM;
+        }
+        if (zero_fill && hipMemset(ptr, 0, want) != hipSuccess) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            return WF_ERR_DEVICE;
+        }
+        cap = want;
+        return WF_OK;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+struct WfArenas {
+    DevBuf codes, pairs, tasks, bnd, res, counters, epoch;
+    void release()
+    {
+        epoch.release();
+        codes.release();
+        pairs.release();
+        tasks.release();
+        bnd.release();
+        res.release();
+        counters.release();
+    }
+};
+
+struct wf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipDeviceProp_t prop{};
+    double timeout_s = 10.0;
+    WfArenas scratch;    // used by wf_align / wf_align_batch
+};
 
 struct WfPlanHost {
     int64_t npairs = 0;
@@ -26,10 +88,19 @@ struct WfPlanHost {
 
 int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
                    uint8_t *out_a, uint8_t *out_b, int32_t *mode);
-const WfVariant *wf_pick_variant(const wf_tile *tile, int64_t max_rows, int64_t total_cells, int npairs);
+const WfVariant *wf_pick_variant(struct wf_ctx *ctx, const wf_tile *tile, const wf_job *jobs, int64_t njobs,
+                                 int64_t max_rows, int64_t total_cells);
+void wf_tune_key(const wf_job *jobs, int64_t njobs, int *problem, int *batch, int *bucket);
+bool wf_tune_lookup(const char *arch, int cus, int problem, int batch, int bucket, int *engine, int *R, int *W,
+                    int *K);
 int wf_default_waves(int64_t max_rows, int64_t total_cells, int npairs);
 int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, int npairs);
 int wf_build_plan(struct wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                   WfPlanHost *ph);
+int wf_upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar);
+int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync);
+int wf_max_resident_groups(wf_ctx *ctx, const WfVariant *v);
+int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int maxg, hipStream_t stream);
+int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms, wf_result *out, int64_t n);
 
 #endif

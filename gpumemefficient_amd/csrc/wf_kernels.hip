@@ -106,7 +106,7 @@ __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, in
 // One strip (`g`) of one pair, swept by the whole workgroup.
 template <int P, bool CMP, int R, int W, int K, int AW>
 __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
-                          WgShared<W, K, AW> &sh, int lane, int wave)
+                          WgShared<W, K, AW> &sh, int lane, int wave, uint32_t epoch)
 {
     constexpr int D = ChunkGeom<K>::D;
     constexpr int RS = ChunkGeom<K>::RS;
@@ -138,7 +138,7 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
     // and no clearing between launches.
     const uint64_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
     uint64_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
-    const uint64_t tag = (uint64_t)ka.epoch << 32;
+    const uint64_t tag = (uint64_t)epoch << 32;
     // Row 0 of the table in this kernel's value representation.
     const int topv = (P == WFK_ED && !(pd.mode & WFK_MODE_TEXTBOOK)) ? -1 : 0;
 
@@ -152,13 +152,13 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
     // ones), then store columns [c*K, c*K+K) into ring[0] and the code window.
     auto fill = [&](int c, uint64_t top, uint32_t code_pre) -> bool {
         const int x = c * K + lane;
-        if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)ka.epoch) != 0) {
+        if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)epoch) != 0) {
             const int64_t t0 = now_ticks();
             for (;;) {
                 __builtin_amdgcn_s_sleep(1);
-                if ((top >> 32) != (uint64_t)ka.epoch)
+                if ((top >> 32) != (uint64_t)epoch)
                     top = __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)ka.epoch) == 0) break;
+                if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)epoch) == 0) break;
                 if (now_ticks() - t0 > ka.timeout_ticks) return false;
             }
         }
@@ -265,6 +265,7 @@ __global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
     __shared__ WgShared<W, K, AW> sh;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const uint32_t epoch = __builtin_amdgcn_readfirstlane(*ka.epoch);
     for (;;) {
         if (threadIdx.x == 0) {
             sh.task = ka.task_begin + atomicAdd(ka.counter, 1);
@@ -278,12 +279,12 @@ __global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
         const WfPairDev pd = ka.pairs[td.pair];
         const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
         switch (pd.prob * 2 + (cmp ? 1 : 0)) {
-        case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        case WFK_SW * 2 + 0: run_group<WFK_SW, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
-        default: run_group<WFK_SW, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave); break;
+        case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_SW * 2 + 0: run_group<WFK_SW, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        default: run_group<WFK_SW, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         }
         __syncthreads();
     }
@@ -291,10 +292,12 @@ __global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
 
 // Zero progress counters, results and claim counters before a launch (a
 // kernel rather than hipMemsetAsync so the sequence can be graph-captured).
-__global__ void wf_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters)
+// Also advances the launch epoch that tags strip-boundary values.
+__global__ void wf_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters, uint32_t *epoch)
 {
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (i0 == 0) *epoch = *epoch + 1u;
     for (int64_t i = i0; i < nres; i += stride) res[i] = WfResultDev{0, 0, 0, 0};
     for (int64_t i = i0; i < ncounters; i += stride) counters[i] = 0;
 }
@@ -320,14 +323,14 @@ extern "C" const WfVariant wf_variants_lockstep[] = {
 extern "C" const int wf_num_variants_lockstep = sizeof(wf_variants_lockstep) / sizeof(wf_variants_lockstep[0]);
 
 extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
-                                      hipStream_t stream)
+                                      uint32_t *epoch, hipStream_t stream)
 {
     int64_t n = nres;
     if (ncounters > n) n = ncounters;
     int blocks = (int)((n + 255) / 256);
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(wfk::wf_reset, dim3(blocks), dim3(256), 0, stream, res, nres, counters, ncounters);
+    hipLaunchKernelGGL(wfk::wf_reset, dim3(blocks), dim3(256), 0, stream, res, nres, counters, ncounters, epoch);
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ extern "C" const WfVariant wf_variants_async[];
 extern "C" const int wf_num_variants_async;
 
 extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
-                                      hipStream_t stream);
+                                      uint32_t *epoch, hipStream_t stream);
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
 

@@ -71,8 +71,8 @@ typedef struct {
 #define WF_TILE_ED_TEXTBOOK 0x1 /* ED row 0 = x (reference GPU_setInit), not ED.cpp's x-1 */
 #define WF_TILE_CACHE 0x2       /* hyperlane-Cache policy: one wave per workgroup, all
                                    strip boundaries through L2 (GPU_Hyperlane_Cache.cu) */
-#define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine instead of the default
-                                   asynchronous one (kept for comparison) */
+#define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine (the default) */
+#define WF_TILE_ASYNC 0x8       /* asynchronous engine: LDS progress counters + I/O wave */
 
 typedef struct {
     int32_t last;      /* bottom-right cell: the score the reference prints */
@@ -129,16 +129,82 @@ enum { WF_RUN_PERSISTENT = 0, WF_RUN_PER_PAIR = 1 };
 int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                    wf_plan **out);
 int wf_plan_run(wf_plan *plan, int mode);                 /* enqueue + wait */
+/* Capture the plan's launches for `mode` into a hipGraph; later
+ * wf_plan_run(plan, mode) calls replay the graph. */
+int wf_plan_capture(wf_plan *plan, int mode);
 int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n);
 double wf_plan_kernel_ms(const wf_plan *plan);            /* last run, HIP events */
 int wf_plan_describe(const wf_plan *plan, int32_t *engine, int32_t *rows_per_lane,
                      int32_t *waves, int32_t *chunk, int64_t *ntasks, int32_t *grid);
+
+/* Autotuner: time every compiled variant on `jobs` (resident in HBM, `reps`
+ * timed runs each), return the fastest as a tile and remember it for
+ * workloads of the same class on this device; plans created without an
+ * explicit tile use the remembered choice.  The table persists through
+ * wf_tune_save / wf_tune_load (text, one entry per line). */
+int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_tile *best,
+                double *best_ms);
+int wf_tune_load(const char *path);
+int wf_tune_save(const char *path);
+int wf_tune_clear(void);
 
 /* Compiled kernel variants (engine 0 = lock-step, 1 = asynchronous). */
 int wf_variant_count(void);
 int wf_variant_info(int index, int32_t *engine, int32_t *rows_per_lane, int32_t *waves,
                     int32_t *chunk);
 void wf_plan_destroy(wf_plan *plan);
+
+/* ------------------------------------------------------------------------
+ * Server: continuous batching of requests that arrive over time.  A
+ * scheduler thread admits queued requests in arrival order while their
+ * device footprint fits the HBM budget and launches every admitted request
+ * not yet on the device as one persistent launch on the next free stream,
+ * so requests that arrive while others run overlap with them on the device.
+ * No counterpart in the reference.
+ * ---------------------------------------------------------------------- */
+typedef struct wf_server wf_server;
+
+typedef struct {
+    int32_t streams;        /* concurrent launches (HIP streams); 0 = 4 */
+    int32_t max_batch;      /* requests per launch; 0 = 512 */
+    int64_t hbm_budget;     /* device bytes the server may hold; 0 = half of free memory */
+    int32_t queue_capacity; /* queued requests before submit returns WF_ERR_AGAIN; 0 = 2^20 */
+    int32_t per_request;    /* 1: one launch per request on one stream (baseline) */
+} wf_server_opts;
+
+typedef struct {
+    uint64_t ticket;
+    int32_t last;
+    int32_t max_score;
+    int32_t status;
+    int32_t reserved;
+    double t_submit; /* seconds on the server clock (replay: scheduled arrival) */
+    double t_start;  /* launch */
+    double t_done;   /* completion observed by the host */
+} wf_completion;
+
+typedef struct {
+    uint64_t launches;
+    uint64_t completed;
+    uint64_t cells;
+    int64_t hbm_budget;
+    int64_t hbm_used;
+    int64_t pending;
+} wf_server_stats_t;
+
+int wf_server_create(int device, const wf_server_opts *opts, wf_server **out);
+/* Copies the job's sequences; returns at once with a ticket. */
+int wf_server_submit(wf_server *s, const wf_job *job, uint64_t *ticket);
+/* Completed requests since the last poll (up to max). */
+int wf_server_poll(wf_server *s, wf_completion *out, int max, int *n);
+int wf_server_wait_idle(wf_server *s, double timeout_s);
+int wf_server_stats(wf_server *s, wf_server_stats_t *st);
+/* Submit jobs[i] at arrival_s[i] seconds after the call (real time, from
+ * the calling thread) and wait for all of them; out[i] gets job i's
+ * completion with times relative to the start. */
+int wf_server_replay(wf_server *s, const wf_job *jobs, const double *arrival_s, int64_t n,
+                     wf_completion *out);
+void wf_server_destroy(wf_server *s);
 
 #ifdef __cplusplus
 }

@@ -133,3 +133,57 @@ def test_plan_persistent_equals_per_pair(ctx):
         assert (x.last, x.max_score) == (last, mx)
         assert (y.last, y.max_score) == (last, mx)
     plan.close()
+
+
+def _mixed_jobs(seed, count, maxlen):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(count):
+        prob = ["LCS", "ED", "SW"][i % 3]
+        n1, n2 = int(rng.integers(1, maxlen)), int(rng.integers(1, maxlen))
+        jobs.append(Job(prob, *rand_pair(rng, n1, n2, 20 if prob == "ED" else 4)))
+    return jobs
+
+
+def _expected(jobs):
+    return [O.score(j.problem, j.a, j.b) for j in jobs]
+
+
+def test_graph_replay_matches_eager(ctx):
+    """hipGraph replays of a captured plan, interleaved with eager work of
+    another plan and host synchronisation, all equal the eager result."""
+    jobs = _mixed_jobs(21, 60, 2500)
+    want = _expected(jobs)
+    other = ctx.plan(_mixed_jobs(22, 10, 800))
+    for per_pair in (False, True):
+        plan = ctx.plan(jobs)
+        plan.run(per_pair=per_pair)
+        eager = [(r.last, r.max_score) for r in plan.results()]
+        assert eager == want
+        plan.capture(per_pair=per_pair)
+        for _ in range(5):
+            plan.run(per_pair=per_pair)
+            assert [(r.last, r.max_score) for r in plan.results()] == want
+            other.run()
+            other.results()
+        plan.close()
+    other.close()
+
+
+def test_autotune_roundtrip(ctx, tmp_path):
+    from gpumemefficient_amd.api import tune_clear, tune_load, tune_save
+    jobs = _mixed_jobs(23, 24, 1200)
+    tile, ms = ctx.autotune(jobs, reps=1)
+    assert ms > 0 and tile.engine in ("lockstep", "async")
+    path = tmp_path / "tune.tsv"
+    tune_save(path)
+    tune_clear()
+    tune_load(path)
+    plan = ctx.plan(jobs)           # no tile: the tuned choice
+    d = plan.describe()
+    assert d["engine"] == tile.engine and d["chunk"] == tile.tile_w
+    assert d["rows_per_lane"] * d["waves"] * 64 == tile.strip_rows
+    plan.run()
+    assert [(r.last, r.max_score) for r in plan.results()] == _expected(jobs)
+    plan.close()
+    tune_clear()
