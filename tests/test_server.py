@@ -1,0 +1,125 @@
+"""Serving path: continuous batching, admission by HBM budget, results
+identical alone and in a mix (bit-exact against the oracle)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpumemefficient_amd import Job
+from gpumemefficient_amd import _lib
+from gpumemefficient_amd.datagen import serving_requests
+from gpumemefficient_amd.server import Server, percentile
+
+
+def _jobs(seed, count, lo, hi):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        prob = "ED" if i % 2 else "SW"
+        alpha = 20 if prob == "ED" else 4
+        n1, n2 = int(rng.integers(lo, hi)), int(rng.integers(lo, hi))
+        out.append(Job(prob, rng.integers(0, alpha, n1, dtype=np.int32), rng.integers(0, alpha, n2, dtype=np.int32)))
+    return out
+
+
+def _want(j):
+    return O.score(j.problem, j.a, j.b)
+
+
+def test_percentile_helper():
+    assert percentile([1, 2, 3, 4], 50) == 2.5
+    assert np.isnan(percentile([], 99))
+
+
+def test_serving_trace_generator():
+    reqs = serving_requests(1000.0, 500, seed=5)
+    assert len(reqs) == 500
+    t = np.array([r.arrival_s for r in reqs])
+    assert np.all(np.diff(t) > 0)
+    rate = len(reqs) / t[-1]
+    assert 800 < rate < 1250  # Poisson at 1k/s
+    probs = {r.problem for r in reqs}
+    assert probs == {"ED", "SW"}
+    for r in reqs:
+        alpha = 20 if r.problem == "ED" else 4
+        assert r.a.max() < alpha and r.b.max() < alpha
+        assert 100 <= r.a.size <= 131072 and 100 <= r.b.size <= 131072
+    # Same seed, same trace.
+    again = serving_requests(1000.0, 500, seed=5)
+    assert all(np.array_equal(x.a, y.a) and x.arrival_s == y.arrival_s for x, y in zip(reqs, again))
+
+
+@pytest.mark.gpu
+def test_server_results_and_alone_vs_mix():
+    jobs = _jobs(31, 120, 50, 3000)
+    want = [_want(j) for j in jobs]
+    with Server(0, streams=4) as srv:
+        # Alone: one request at a time.
+        alone = []
+        for j in jobs[:20]:
+            srv.submit(j)
+            srv.wait_idle(60)
+            alone += srv.poll()
+        # In the mix: everything at once, overlapping on the device.
+        tickets = [srv.submit(j) for j in jobs]
+        srv.wait_idle(120)
+        mix = {c.ticket: c for c in srv.poll()}
+        st = srv.stats()
+    assert len(alone) == 20 and all(c.status == 0 for c in alone)
+    for c, w in zip(alone, want[:20]):
+        assert (c.last, c.max_score) == w
+    for t, w, j in zip(tickets, want, jobs):
+        c = mix[t]
+        assert c.status == 0
+        assert (c.last, c.max_score) == w, (j.problem, j.a.size, j.b.size)
+        assert c.t_submit <= c.t_start <= c.t_done
+    assert st["completed"] == 140 and st["hbm_used"] == 0
+
+
+@pytest.mark.gpu
+def test_server_admission_by_budget():
+    """A tight HBM budget forces requests to wait for earlier ones to
+    finish; all still complete correctly.  A request that can never fit
+    fails with WF_ERR_NOMEM instead of blocking the queue."""
+    jobs = _jobs(32, 30, 1000, 4000)
+    want = [_want(j) for j in jobs]
+    with Server(0, streams=2, hbm_budget=600_000) as srv:
+        tickets = [srv.submit(j) for j in jobs]
+        big = Job("SW", np.zeros(40000, np.int32), np.zeros(40000, np.int32))
+        tb = srv.submit(big)
+        srv.wait_idle(120)
+        got = {c.ticket: c for c in srv.poll()}
+        st = srv.stats()
+    for t, w in zip(tickets, want):
+        assert got[t].status == 0 and (got[t].last, got[t].max_score) == w
+    assert got[tb].status == _lib.WF_ERR_NOMEM
+    assert st["launches"] >= 3  # could not all go in one launch
+
+
+@pytest.mark.gpu
+def test_server_replay_poisson():
+    reqs = serving_requests(2000.0, 150, seed=7)
+    rng = np.random.default_rng(1)
+    jobs = []
+    for r in reqs:  # cap lengths so the oracle check stays fast
+        a, b = r.a[:2500], r.b[:2500]
+        jobs.append(Job(r.problem, a, b))
+    arrivals = [r.arrival_s for r in reqs]
+    with Server(0) as srv:
+        comps = srv.replay(jobs, arrivals)
+    for c, j, t in zip(comps, jobs, arrivals):
+        assert c.status == 0
+        assert (c.last, c.max_score) == _want(j)
+        assert c.t_submit == t and c.t_done >= c.t_submit
+    lat = [c.latency for c in comps]
+    assert percentile(lat, 99) < 5.0
+
+
+@pytest.mark.gpu
+def test_server_per_request_baseline_matches():
+    jobs = _jobs(33, 20, 50, 2000)
+    with Server(0, per_request=True) as srv:
+        comps = srv.replay(jobs, [0.0] * len(jobs))
+        st = srv.stats()
+    assert st["launches"] == len(jobs)
+    for c, j in zip(comps, jobs):
+        assert (c.last, c.max_score) == _want(j)
