@@ -18,8 +18,10 @@ One JSON line on stdout:
   cpu_baseline the oracle (C restatement of ED.cpp, one thread) timed on this
                host on the same pair
   extra        the other headline measurements: Smith-Waterman 131072^2
-               single pair, and the 4096-pair batch against the loop of one
-               launch per pair
+               single pair; the 4096-pair batch against the loop of one
+               launch per pair; the served Poisson stream (p50/p99 latency
+               and GCUPS per rate, continuous batching against one launch
+               per request)
 """
 from __future__ import annotations
 
@@ -42,17 +44,26 @@ def peak_int32_tops(info: dict) -> float:
     return info["compute_units"] * 64 * info["max_clock_mhz"] * 1e6 / 1e12
 
 
-def traffic_for(workload: str):
+def kernel_name(desc: dict) -> str:
+    base = "wf_engine" if desc["engine"] == "lockstep" else "wf_engine_async"
+    return f"{base}<{desc['rows_per_lane']},{desc['waves']},{desc['chunk']}>"
+
+
+def traffic_for(workload: str, desc: dict):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), as committed in profiles/."""
+    scaled by the copy-kernel calibration), as recorded in
+    profiles/traffic.json -- only if it was measured on the kernel variant
+    this run uses."""
     f = PROFILES / "traffic.json"
     if not f.exists():
         return None
     try:
-        d = json.loads(f.read_text())
-        return d.get(workload, {}).get("hbm_bytes_per_launch")
-    except Exception:
+        d = json.loads(f.read_text()).get(workload, {})
+    except (OSError, ValueError):
         return None
+    if d.get("kernel") != kernel_name(desc):
+        return None
+    return d.get("hbm_bytes_per_launch")
 
 
 def make_workload(name: str):
@@ -115,7 +126,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ed", choices=["ed", "sw", "lcs", "batch"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the SW single-pair and batch measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the SW single-pair, batch and serving measurements")
+    ap.add_argument("--serve-requests", type=int, default=600, help="requests per rate in the serving measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -159,7 +171,8 @@ def main() -> int:
             "peak": round(peak, 3),
             "unit": "TOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": traffic_for(args.workload),
+            "traffic": traffic_for(args.workload, desc),
+            "kernel": kernel_name(desc),
             "ops_per_cell": OPS_PER_CELL[prob],
             "kernel_ms": round(kernel_ms, 4),
         },
@@ -194,6 +207,17 @@ def main() -> int:
                 "per_pair_launch_gcups": round(bcells / pk / 1e6, 1), "per_pair_launch_ms": round(pk, 3),
                 "speedup": round(pk / bk, 2), "engine": p.describe()}
             p.close()
+        # Served mixed workload (Poisson arrivals, 50% ED / 50% SW, mixed
+        # lengths), continuous batching against one launch per request.
+        from gpumemefficient_amd.serve_bench import run_rate
+        serve = {"requests_per_rate": args.serve_requests, "continuous": [], "per_request": []}
+        for rate in (1000.0, 10000.0, 50000.0):
+            serve["continuous"].append(run_rate(rate, args.serve_requests, False))
+        serve["per_request"].append(run_rate(50000.0, args.serve_requests, True))
+        sat_c = serve["continuous"][-1]["gcups"]
+        sat_p = serve["per_request"][-1]["gcups"]
+        serve["saturated_speedup"] = round(sat_c / sat_p, 2) if sat_p else None
+        extra["serving"] = serve
         line["extra"] = extra
 
     if not args.no_cpu:

@@ -152,6 +152,19 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
             if (const WfVariant *v = find_variant(te, tr, tw, tk)) return v;
         }
     }
+    // Default engine: a launch whose strips all fit on the chip at once is
+    // bound by the wavefront's critical path, where the asynchronous engine's
+    // shorter lag between waves and strips wins; a launch with many more
+    // strips than CUs is bound by throughput, where the lock-step engine's
+    // higher occupancy wins (measured on MI355X: SW 131072^2 and ED 65536^2
+    // against the 4096-pair batch).
+    {
+        int64_t strips = 0;
+        for (int64_t i = 0; i < njobs; ++i)
+            if (jobs[i].n1 > 0 && jobs[i].n2 > 0) strips += (jobs[i].n2 + 511) / 512;
+        const int cus = ctx ? ctx->prop.multiProcessorCount : 256;
+        if (strips <= 2LL * cus) E = WF_ENGINE_ASYNC;
+    }
     if (tile && (tile->flags & WF_TILE_ASYNC)) E = WF_ENGINE_ASYNC;
     if (tile && (tile->flags & WF_TILE_LOCKSTEP)) E = WF_ENGINE_LOCKSTEP;
     if (cache) E = WF_ENGINE_ASYNC;
@@ -180,7 +193,7 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
     }
     if (W == 0) W = cache ? 1 : wf_default_waves(max_rows, total_cells, (int)njobs);
     if (R == 0) R = wf_default_rows_per_lane(W, max_rows, total_cells, (int)njobs);
-    if (K == 0) K = (E == WF_ENGINE_LOCKSTEP) ? 64 : 32;
+    if (K == 0) K = (E == WF_ENGINE_LOCKSTEP) ? 64 : 16;
     // Snap to the nearest compiled variant of the requested engine.
     const WfVariant *best = nullptr;
     int best_d = 1 << 30;
@@ -279,11 +292,31 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         p.mode = mode;
     }
 
-    // Tasks: largest pairs first; the strips of a pair stay contiguous and in
-    // order, which is what makes the persistent claim loop deadlock-free.
+    // Tasks: pairs with the longest dependency chain first (columns plus the
+    // start-up lag of every strip), then by size; the strips of a pair stay
+    // contiguous and in order, which is what makes the persistent claim loop
+    // deadlock-free.  The longest chains also get the highest wave issue
+    // priority, so that co-resident strips of short pairs fill the issue
+    // slots they leave idle instead of stretching the critical path.
+    std::vector<int64_t> chain(njobs, 0);
+    int64_t max_chain = 1;
+    const int64_t strip_lag = (int64_t)v->waves * (64 + 2 * v->chunk);
+    for (int64_t i = 0; i < njobs; ++i) {
+        const WfPairDev &p = ph->pairs[i];
+        if (p.ngroups == 0) continue;
+        chain[i] = p.n1 + 63 + (int64_t)p.ngroups * strip_lag;
+        max_chain = std::max(max_chain, chain[i]);
+    }
+    static const bool use_prio = [] {
+        const char *e = std::getenv("WFDP_PRIORITY");
+        return !(e && e[0] == '0');
+    }();
+    for (int64_t i = 0; i < njobs; ++i)
+        ph->pairs[i].prio = use_prio ? (int32_t)std::min<int64_t>(3, 4 * chain[i] / (max_chain + 1)) : 0;
     std::vector<int64_t> order(njobs);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        if (chain[x] != chain[y]) return chain[x] > chain[y];
         return ph->cells[x] > ph->cells[y];
     });
     ph->tasks.clear();
@@ -353,8 +386,6 @@ int wf_max_resident_groups(wf_ctx *ctx, const WfVariant *v)
 // the epoch that tags boundary values is advanced on the device).
 int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int maxg, hipStream_t stream)
 {
-    const WfVariant *v = ph->variant;
-    const int64_t ntasks = (int64_t)ph->tasks.size();
     const int64_t ncounters = ph->npairs + 1;
     if (wf_launch_reset((WfResultDev *)ar->res.ptr, ph->npairs, (int32_t *)ar->counters.ptr, ncounters,
                         (uint32_t *)ar->epoch.ptr, stream) != hipSuccess)
@@ -368,7 +399,9 @@ int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int max
     ka.epoch = (const uint32_t *)ar->epoch.ptr;
     ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
     ph->grid = 0;
-    if (ntasks == 0) return WF_OK;
+    if (ph->tasks.empty()) return WF_OK;
+    const WfVariant *v = ph->variant;
+    const int lim = maxg > 0 ? maxg : wf_max_resident_groups(ctx, v);
     if (mode == WF_RUN_PER_PAIR) {
         for (int64_t k = 0; k < ph->npairs; ++k) {
             const int64_t i = ph->order[k];
@@ -377,18 +410,17 @@ int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int max
             ka.counter = (int32_t *)ar->counters.ptr + 1 + i;
             ka.task_begin = (int32_t)tb;
             ka.task_end = (int32_t)te;
-            const int grid = (int)std::min<int64_t>(te - tb, maxg);
+            const int grid = (int)std::min<int64_t>(te - tb, lim);
             ph->grid = std::max(ph->grid, grid);
             if (wf_launch_engine(v, grid, &ka, stream) != hipSuccess) return WF_ERR_DEVICE;
         }
-    } else {
-        ka.counter = (int32_t *)ar->counters.ptr;
-        ka.task_begin = 0;
-        ka.task_end = (int32_t)ntasks;
-        const int grid = (int)std::min<int64_t>(ntasks, maxg);
-        ph->grid = grid;
-        if (wf_launch_engine(v, grid, &ka, stream) != hipSuccess) return WF_ERR_DEVICE;
+        return WF_OK;
     }
+    ka.counter = (int32_t *)ar->counters.ptr;
+    ka.task_begin = 0;
+    ka.task_end = (int32_t)ph->tasks.size();
+    ph->grid = (int)std::min<int64_t>((int64_t)ph->tasks.size(), lim);
+    if (wf_launch_engine(v, ph->grid, &ka, stream) != hipSuccess) return WF_ERR_DEVICE;
     return WF_OK;
 }
 

@@ -115,6 +115,17 @@ __device__ __forceinline__ void step(LaneState<R> &st, int inj, uint32_t ca, boo
     st.dg = MASKED ? (valid ? up0 : st.dg) : up0;
 }
 
+// Issue priority of the calling wave (s_setprio takes an immediate).
+__device__ __forceinline__ void set_wave_priority(int p)
+{
+    switch (p) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(0); break;
+    }
+}
+
 __device__ __forceinline__ int64_t now_ticks()
 {
     return (int64_t)__builtin_amdgcn_s_memrealtime();

@@ -22,7 +22,7 @@ struct WfPairDev {
     int32_t n1, n2;
     int32_t prob;      // WFK_LCS / WFK_ED / WFK_SW
     int32_t mode;      // WFK_MODE_* bits
-    int32_t pad;
+    int32_t prio;      // wave issue priority (s_setprio 0..3) of this pair's strips
     int32_t ngroups;   // strips of (waves * 64 * rows_per_lane) rows
 };
 

@@ -1,4 +1,4 @@
-// wf_engine.hip -- asynchronous strip engine for gfx950 (the default engine).
+// wf_engine.hip -- asynchronous strip engine for gfx950.
 //
 // Same lane geometry as the lock-step engine in wf_kernels.hip (lane l owns
 // R rows and computes column s - l at step s; DPP wave_shr:1 passes the
@@ -31,6 +31,23 @@
 
 namespace wfk {
 
+#ifdef WFK_TRACE
+// Diagnostic build only (-DWFK_TRACE): per (task, wave) timestamps of the
+// strip schedule, read back with wf_trace_read_async().
+struct TraceRec {
+    int64_t t_begin, t_start, t_end;   // s_memrealtime (100 MHz)
+    int64_t wait_cyc, tot_cyc;         // s_memtime cycles
+    int32_t nwait, hwid, xcc, pad;
+};
+constexpr int kTraceTasks = 8192, kTraceWaves = 9;
+__device__ TraceRec g_trace[kTraceTasks * kTraceWaves];
+#endif
+
+// Steps per iteration of the steady compute loop (a multiple of the check
+// granularity G): one loop back-edge per kU steps, the counter checks in
+// between are not-taken branches.
+constexpr int kU = 32;
+
 template <int W, int G, int AW, int RS>
 struct AsyncShared {
     // ring[0]: the strip's top input, staged by the I/O wave; ring[w+1]:
@@ -39,9 +56,11 @@ struct AsyncShared {
     __attribute__((aligned(16))) int ring[W + 1][RS];
     // Lanes 0..62 store their bottom cell here so the capture is one
     // unmasked ds_write per step.
-    int dummy[W][64 + G];
-    // Column codes: column x at x mod AW, first G slots mirrored at AW.
-    uint8_t aw[AW + G];
+    int dummy[W][64 + kU];
+    // Column codes, one per dword (a lane's 4 codes of a block are then 4
+    // aligned dword reads, which LDS serves faster than unaligned or byte
+    // reads): column x at x mod AW, first G slots mirrored at AW.
+    uint32_t aw[AW + kU + 4];
     // Progress counters, 64 copies each so a wave stores one value with an
     // unmasked ds_write:
     //   cnt[0]     columns staged in ring[0] and aw
@@ -66,9 +85,13 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 // can only be seen once those stores are visible, and ring loads issued
 // after a counter load returned see them; only the compiler has to be kept
 // from moving the accesses across the counter (a signal fence).  Building
-// with -DWFK_LDS_HW_FENCE=1 uses s_waitcnt-based workgroup fences instead.
+// with -DWFK_LDS_HW_FENCE=1 uses s_waitcnt-based workgroup fences instead
+// (which drain the read pipeline at every group boundary).
 #ifndef WFK_LDS_HW_FENCE
-#define WFK_LDS_HW_FENCE 1
+#define WFK_LDS_HW_FENCE 0
+#endif
+#ifndef WFK_IO_PRIO
+#define WFK_IO_PRIO -1
 #endif
 __device__ __forceinline__ void lds_release()
 {
@@ -87,25 +110,77 @@ __device__ __forceinline__ void lds_acquire()
 #endif
 }
 
-// One group of G steps starting at s0 (a multiple of G) for one compute wave.
-template <int P, bool CMP, int R, int G, int AW, int RS, bool MASKED, bool TRACK>
-__device__ __forceinline__ void run_steps(LaneState<R> &st, int s0, int lane, int n1, const int *ring_prev,
-                                          int *ring_cur, int *dummy, const uint8_t *aw)
+// Operands of one 4-step block: lane 0's top neighbours of 4 columns (one
+// broadcast ds_read_b128) and the lane's 4 column codes.
+struct Block4 {
+    int4 inj;
+    uint32_t ca[4];
+};
+
+template <int AW, int RS>
+__device__ __forceinline__ void load_block(Block4 &b, const int *ring_prev, const uint32_t *aw, int s, int lane)
 {
-    const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (RS - 1));
-    const uint8_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
+    b.inj = *reinterpret_cast<const int4 *>(ring_prev + ((s + 64) & (RS - 1)));
+    const uint32_t *cp = aw + ((s - lane) & (AW - 1));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b.ca[u] = cp[u];
+}
+
+// U steps of one compute wave, starting at step q0*G (U a multiple of G; the
+// progress counters work in groups of G steps).  `blk` holds the first
+// block (4 steps) on entry and the block after the last on return.  The LDS
+// reads of block k+1 are issued before block k is computed.  At the last
+// block of each group the counters are checked for the next group: its
+// inputs must be there (producer) and its output slots free (consumer).
+// The counters checked were loaded at the previous group's last block
+// (`vin`/`vout`, reloaded right after the check), so the check does not wait
+// on a load, and the reads after it see everything the counters vouch for
+// (LDS executes one wave's accesses in order).  A short counter sends the
+// wave to `wait_next`, out of line.  The group's progress is published after
+// its last block.  There is one load site per block, so the loop-carried
+// block stays in the same VGPRs.  Returns false if a wait was abandoned.
+template <int P, bool CMP, int R, int G, int U, int AW, int RS, bool MASKED, bool TRACK, class NeedIn,
+          class NeedOut, class WaitFn, class PubFn>
+__device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int lane, int n1, const int *ring_prev,
+                                          int *ring_cur, int *dummy, const uint32_t *aw, Block4 &blk,
+                                          const int *cin, const int *cout, int &vin, int &vout,
+                                          NeedIn &&need_in, NeedOut &&need_out, WaitFn &&wait_next,
+                                          PubFn &&publish)
+{
+    static_assert(U % G == 0 && G % 4 == 0, "bad unroll");
+    const int s0 = q0 * G;
+    // s0 is a multiple of U, so neither the ring nor the (mirrored) code
+    // window wraps inside the iteration; only the next iteration's first
+    // block is addressed afresh.
+    const int *__restrict__ ib = ring_prev + ((s0 + 64) & (RS - 1));
+    const uint32_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
     int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
-    int *cap_last = (lane == 63) ? ring_cur + ((s0 + G) & (RS - 1)) : dummy + lane + (G - 1);
-    int4 inj_n = *reinterpret_cast<const int4 *>(inj_base);
-    uint32_t ca_n[4] = {cp[0], cp[1], cp[2], cp[3]};
+    int *cap_last = (lane == 63) ? ring_cur + ((s0 + U) & (RS - 1)) : dummy + lane + (U - 1);
+    bool ok = true;
 #pragma unroll
-    for (int q = 0; q < G; q += 4) {
-        const int injv[4] = {inj_n.x, inj_n.y, inj_n.z, inj_n.w};
-        const uint32_t cav[4] = {ca_n[0], ca_n[1], ca_n[2], ca_n[3]};
-        if (q + 4 < G) {
-            inj_n = *reinterpret_cast<const int4 *>(inj_base + q + 4);
+    for (int q = 0; q < U; q += 4) {
+        const int injv[4] = {blk.inj.x, blk.inj.y, blk.inj.z, blk.inj.w};
+        const uint32_t cav[4] = {blk.ca[0], blk.ca[1], blk.ca[2], blk.ca[3]};
+        const int gi = q0 + q / G;  // group of this block
+        if (q % G == G - 4) {
+            // Wave-uniform: every lane reads its own copy of the counters.
+            const int slack = min(__builtin_amdgcn_readfirstlane(vin) - need_in(gi + 1),
+                                  __builtin_amdgcn_readfirstlane(vout) - need_out(gi + 1));
+            if (__builtin_expect(slack < 0 && gi + 1 < ngr, 0)) ok = ok && wait_next(gi + 1);
+            lds_acquire();
+            vin = cnt_load(cin);
+            vout = cnt_load(cout);
+        }
+        if (q + 4 < U) {
+            blk.inj = *reinterpret_cast<const int4 *>(ib + q + 4);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) ca_n[u] = cp[q + 4 + u];
+            for (int u = 0; u < 4; ++u) blk.ca[u] = cp[q + 4 + u];
+        } else {
+            // Past the last group this reads stale slots (in bounds) that
+            // are never used.
+            blk.inj = *reinterpret_cast<const int4 *>(ring_prev + ((s0 + U + 64) & (RS - 1)));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) blk.ca[u] = cp[U + u];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -113,25 +188,55 @@ __device__ __forceinline__ void run_steps(LaneState<R> &st, int s0, int lane, in
             bool valid = true;
             if (MASKED) valid = (unsigned)(s0 + j - lane) < (unsigned)n1;
             step<P, CMP, R, MASKED, TRACK>(st, injv[u], cav[u], valid);
-            if (j == G - 1)
+            if (j == U - 1)
                 *cap_last = st.h[R - 1];
             else
                 cap[j] = st.h[R - 1];
         }
+        if (q % G == G - 4) publish(gi);
     }
+    return ok;
 }
 
 template <int P, bool CMP, int R, int W, int G, int AW, int RS>
 __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
-                            AsyncShared<W, G, AW, RS> &sh, int lane, int wave, uint32_t epoch)
+                            AsyncShared<W, G, AW, RS> &sh, int lane, int wave, uint32_t epoch, int task)
 {
+#ifdef WFK_TRACE
+    const int64_t tr_begin = __builtin_amdgcn_s_memrealtime();
+    const int64_t tr_c0 = clock64();
+    int64_t tr_start = 0, tr_wait = 0;
+    int tr_nwait = 0;
+    auto tr_store = [&]() {
+        if (lane == 0 && task < kTraceTasks) {
+            TraceRec &r = g_trace[task * kTraceWaves + wave];
+            r.t_begin = tr_begin;
+            r.t_start = tr_start;
+            r.t_end = __builtin_amdgcn_s_memrealtime();
+            r.wait_cyc = tr_wait;
+            r.tot_cyc = clock64() - tr_c0;
+            r.nwait = tr_nwait;
+            r.hwid = __builtin_amdgcn_s_getreg((31 << 11) | 4);
+            r.xcc = __builtin_amdgcn_s_getreg((31 << 11) | 20);
+        }
+    };
+#endif
     constexpr bool TRACK = (P == WFK_SW);
     const int n1 = pd.n1, n2 = pd.n2;
     const int nsteps = n1 + 63;
     const bool has_prev = g > 0;
     const bool has_next = g + 1 < pd.ngroups;
     const uint64_t tag = (uint64_t)epoch << 32;
+    // Compute waves holding at least one row of the pair (fewer than W only
+    // in a pair's last strip, which has no consumer below it).
+    const int64_t rows_left = (int64_t)n2 - (int64_t)g * (W * 64 * R);
+    const int wa = (int)min<int64_t>(W, (rows_left + 64 * R - 1) / (64 * R));
 
+    if (wave < W && wave >= wa) {
+        // Only padding rows: never block the wave above (its consumer).
+        cnt_store(sh.cnt[wave + 1] + lane, 1 << 30);
+        return;
+    }
     if (wave < W) {
         // ------------------------------------------------------ compute wave
         const int w = wave;
@@ -159,41 +264,71 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         // which the consumer must have read.
         auto need_out = [&](int q) { return (q + 1) * G - 63 - RS; };
 
-        int vin = cnt_load(cin), vout = cnt_load(cout);
-        bool aborted = false;
-        for (int q = 0; q < ngr; ++q) {
-            if (vin < need_in(q) || vout < need_out(q)) {
+        // Wait until group q's inputs are there and its output slots free.
+        // Every lane holds its own copy of a counter, so the values are read
+        // through readfirstlane and the whole loop stays wave-uniform
+        // (scalar branches, no exec-mask bookkeeping per group).
+        auto ready = [&](int q) -> bool {
+            const int vi = __builtin_amdgcn_readfirstlane(cnt_load(cin));
+            const int vo = __builtin_amdgcn_readfirstlane(cnt_load(cout));
+            return vi >= need_in(q) && vo >= need_out(q);
+        };
+        auto wait_ready = [&](int q) -> bool {
+            if (!ready(q)) {
+#ifdef WFK_TRACE
+                const int64_t c0 = clock64();
+                ++tr_nwait;
+                struct Acc {
+                    int64_t c0, *acc;
+                    __device__ ~Acc() { *acc += clock64() - c0; }
+                } acc{c0, &tr_wait};
+#endif
                 const int64_t t0 = now_ticks();
                 for (;;) {
                     __builtin_amdgcn_s_sleep(1);
-                    vin = cnt_load(cin);
-                    vout = cnt_load(cout);
-                    if (vin >= need_in(q) && vout >= need_out(q)) break;
-                    if (__hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
-                        now_ticks() - t0 > ka.timeout_ticks) {
-                        aborted = true;
-                        break;
-                    }
+                    if (ready(q)) break;
+                    if (__builtin_amdgcn_readfirstlane(
+                            __hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ||
+                        now_ticks() - t0 > ka.timeout_ticks)
+                        return false;
                 }
-                if (aborted) break;
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            // Counters for the next group's check, read now so the latency
-            // hides behind this group's arithmetic.
-            const int vin_n = cnt_load(cin), vout_n = cnt_load(cout);
-            const int s0 = q * G;
-            const bool steady = (s0 >= 63) && (s0 + G - 1 <= n1 - 1);
-            if (steady)
-                run_steps<P, CMP, R, G, AW, RS, false, TRACK>(st, s0, lane, n1, sh.ring[w], sh.ring[w + 1],
-                                                             sh.dummy[w], sh.aw);
-            else
-                run_steps<P, CMP, R, G, AW, RS, true, TRACK>(st, s0, lane, n1, sh.ring[w], sh.ring[w + 1],
-                                                            sh.dummy[w], sh.aw);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            cnt_store(cmine, min(nsteps, s0 + G));
-            vin = vin_n;
-            vout = vout_n;
-        }
+            lds_acquire();
+            return true;
+        };
+        Block4 blk;
+        bool aborted = !wait_ready(0);
+#ifdef WFK_TRACE
+        tr_start = __builtin_amdgcn_s_memrealtime();
+        tr_nwait = 0;
+        tr_wait = 0;
+#endif
+        load_block<AW, RS>(blk, sh.ring[w], sh.aw, 0, lane);
+        int vin = cnt_load(cin), vout = cnt_load(cout);
+        auto publish = [&](int gi) {
+            lds_release();
+            cnt_store(cmine, min(nsteps, (gi + 1) * G));
+        };
+        auto wait_next = [&](int gi) { return wait_ready(gi); };
+        // Steps whose columns are valid for every lane run the unmasked body,
+        // kU steps per loop iteration; the masked ones (lanes before column
+        // 0 at the start, past column n1 - 1 at the end) run one group per
+        // iteration.  One body per loop keeps `blk` in fixed registers.
+        constexpr int UG = kU / G;
+        auto steady_from = [&](int q) { return q * G >= 63 && q * G + kU - 1 <= n1 - 1 && q + UG <= ngr; };
+        int q = 0;
+        for (; !aborted && q < ngr && !(steady_from(q) && (q * G) % kU == 0); ++q)
+            aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
+                st, q, ngr, lane, n1, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
+                need_in, need_out, wait_next, publish);
+        for (; !aborted && steady_from(q); q += UG)
+            aborted = !run_steps<P, CMP, R, G, kU, AW, RS, false, TRACK>(
+                st, q, ngr, lane, n1, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
+                need_in, need_out, wait_next, publish);
+        for (; !aborted && q < ngr; ++q)
+            aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
+                st, q, ngr, lane, n1, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
+                need_in, need_out, wait_next, publish);
         if (aborted) {
             if (lane == 0) __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
@@ -214,17 +349,25 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                 if (lane == 0 && m > 0) atomicMax(&ka.res[pair].maxs, m);
             }
         }
+#ifdef WFK_TRACE
+        tr_store();
+#endif
         return;
     }
 
     // -------------------------------------------------------------- I/O wave
+#if WFK_IO_PRIO >= 0
+    // Below the compute waves: the I/O wave shares a SIMD with one of them
+    // and should only take issue slots that wave leaves idle.
+    __builtin_amdgcn_s_setprio(WFK_IO_PRIO);
+#endif
     const uint8_t *acodes = ka.codes + pd.a_off;
     const uint64_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
     uint64_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
     // Row 0 of the table in the kernels' value representation.
     const int topv = (P == WFK_ED && !(pd.mode & WFK_MODE_TEXTBOOK)) ? -1 : 0;
     const int *c0 = sh.cnt[1] + lane;       // compute wave 0: reads ring[0] and aw
-    const int *clast = sh.cnt[W] + lane;    // compute wave W-1: slowest reader of aw, producer of ring[W]
+    const int *clast = sh.cnt[wa] + lane;   // last compute wave: slowest reader of aw, producer of ring[W]
 
     int in_next = 0;    // columns staged and published in cnt[0]
     int out_done = 0;   // bottom-row columns stored and published in cnt[W+1]
@@ -277,8 +420,8 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                     if (lane >= staged && lane < ready && x < n1) {
                         sh.ring[0][(x + 64) & (RS - 1)] = has_prev ? (int)(uint32_t)top[0] : topv;
                         const int slot = x & (AW - 1);
-                        sh.aw[slot] = (uint8_t)code[0];
-                        if (slot < G) sh.aw[AW + slot] = (uint8_t)code[0];
+                        sh.aw[slot] = code[0];
+                        if (slot < kU + 4) sh.aw[AW + slot] = code[0];
                     }
                     staged = ready;
                     lds_release();
@@ -335,6 +478,9 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
             for (int x = out_done + lane; x < n1; x += 64)
                 __hip_atomic_store(bnext + x, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+#ifdef WFK_TRACE
+    tr_store();
+#endif
 }
 
 template <int R, int W, int G>
@@ -351,7 +497,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void wf_engine_async(WfKernelArgs ka)
     constexpr int RS = AsyncCfg<R, W, G>::RS;
     __shared__ AsyncShared<W, G, AW, RS> sh;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t epoch = __builtin_amdgcn_readfirstlane(*ka.epoch);
     for (;;) {
         if (threadIdx.x == 0) sh.task = ka.task_begin + atomicAdd(ka.counter, 1);
@@ -359,18 +505,19 @@ __global__ __launch_bounds__(64 * (W + 1)) void wf_engine_async(WfKernelArgs ka)
         for (int i = threadIdx.x; i < (W + 2) * 64; i += blockDim.x) (&sh.cnt[0][0])[i] = 0;
         if (threadIdx.x == 0) sh.abort = 0;
         __syncthreads();
-        const int t = sh.task;
+        const int t = __builtin_amdgcn_readfirstlane(sh.task);
         if (t >= ka.task_end) break;
         const WfTaskDev td = ka.tasks[t];
         const WfPairDev pd = ka.pairs[td.pair];
+        set_wave_priority(pd.prio);
         const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
         switch (pd.prob * 2 + (cmp ? 1 : 0)) {
-        case WFK_LCS * 2 + 0: async_strip<WFK_LCS, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_LCS * 2 + 1: async_strip<WFK_LCS, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_ED * 2 + 0: async_strip<WFK_ED, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_ED * 2 + 1: async_strip<WFK_ED, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_SW * 2 + 0: async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        default: async_strip<WFK_SW, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_LCS * 2 + 0: async_strip<WFK_LCS, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
+        case WFK_LCS * 2 + 1: async_strip<WFK_LCS, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
+        case WFK_ED * 2 + 0: async_strip<WFK_ED, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
+        case WFK_ED * 2 + 1: async_strip<WFK_ED, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
+        case WFK_SW * 2 + 0: async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
+        default: async_strip<WFK_SW, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         }
         __syncthreads();
     }
@@ -389,3 +536,10 @@ extern "C" const WfVariant wf_variants_async[] = {
     WF_ASYNC(1, 8, 16), WF_ASYNC(2, 8, 16), WF_ASYNC(2, 3, 16), WF_ASYNC(2, 7, 16),
 };
 extern "C" const int wf_num_variants_async = sizeof(wf_variants_async) / sizeof(wf_variants_async[0]);
+
+#ifdef WFK_TRACE
+extern "C" hipError_t wf_trace_read_async(void *host, size_t bytes)
+{
+    return hipMemcpyFromSymbol(host, HIP_SYMBOL(wfk::g_trace), bytes, 0, hipMemcpyDeviceToHost);
+}
+#endif

@@ -53,9 +53,11 @@ struct WgShared {
     // Lanes 0..62 store their bottom cell here so that the capture store is
     // one unmasked ds_write per step; only lane 63's store lands in a ring.
     int dummy[W][64 + K];
-    // Window of column codes: column x lives at x mod AW, the first K slots
-    // are mirrored after the end so a chunk never wraps.
-    uint8_t aw[AW + K];
+    // Window of column codes, one per dword (aligned dword reads are the
+    // cheapest LDS reads for the lane-skewed access): column x lives at
+    // x mod AW, the first K slots are mirrored after the end so a chunk never
+    // wraps.
+    uint32_t aw[AW + K];
     int task;
     // Chunk index at which wave 0 hit a dependency timeout (INT_MAX: none).
     // Waves test `abort_at <= j` after barrier j; wave 0 can only store a
@@ -68,11 +70,11 @@ struct WgShared {
 template <int P, bool CMP, int R, int K, int AW, bool MASKED, bool TRACK>
 __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, int n1,
                                           const int *ring_prev, int *ring_cur, int *dummy,
-                                          const uint8_t *aw)
+                                          const uint32_t *aw)
 {
     constexpr int RS = ChunkGeom<K>::RS;
     const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (RS - 1));
-    const uint8_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
+    const uint32_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
     int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
     int *cap_last = (lane == 63) ? ring_cur + ((s0 + K) & (RS - 1)) : dummy + lane + (K - 1);
     // Software pipeline: the LDS reads of group q+1 (4 column codes and the
@@ -129,7 +131,11 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
 
     const int nsteps = n1 + 63;
     const int nch = (nsteps + K - 1) / K;
-    const int total = nch + (W - 1) * D;
+    // Waves holding at least one row of the pair (fewer than W only in a
+    // pair's last strip): the strip ends when the last of them is done.
+    const int64_t rows_left = (int64_t)n2 - (int64_t)g * (W * 64 * R);
+    const int wa = (int)min<int64_t>(W, (rows_left + 64 * R - 1) / (64 * R));
+    const int total = nch + (wa - 1) * D;
     const bool has_prev = g > 0;
     const bool has_next = g + 1 < pd.ngroups;
     // Strip-boundary rows hold (launch epoch << 32 | value): a reader knows
@@ -166,8 +172,8 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
             const int v = has_prev ? (int)(uint32_t)top : topv;
             sh.ring[0][(x + 64) & (RS - 1)] = v;
             const int slot = x & (AW - 1);
-            sh.aw[slot] = (uint8_t)code_pre;
-            if (slot < K) sh.aw[AW + slot] = (uint8_t)code_pre;
+            sh.aw[slot] = code_pre;
+            if (slot < K) sh.aw[AW + slot] = code_pre;
         }
         return true;
     };
@@ -202,7 +208,7 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
             code_pre = (lane < K && x < n1) ? acodes[x] : 0u;
             top_pre = load_top(j + 1);
         }
-        if (cj >= 0 && cj < nch) {
+        if (cj >= 0 && cj < nch && wave < wa) {
             const int s0 = cj * K;
             const bool steady = (s0 >= 63) && (s0 + K - 1 <= n1 - 1);
             if (steady)
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
     constexpr int AW = EngineCfg<R, W, K>::AW;
     __shared__ WgShared<W, K, AW> sh;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t epoch = __builtin_amdgcn_readfirstlane(*ka.epoch);
     for (;;) {
         if (threadIdx.x == 0) {
@@ -272,11 +278,12 @@ __global__ __launch_bounds__(64 * W) void wf_engine(WfKernelArgs ka)
             sh.abort_at = 0x7fffffff;
         }
         __syncthreads();
-        const int t = sh.task;
+        const int t = __builtin_amdgcn_readfirstlane(sh.task);
         __syncthreads();
         if (t >= ka.task_end) break;
         const WfTaskDev td = ka.tasks[t];
         const WfPairDev pd = ka.pairs[td.pair];
+        set_wave_priority(pd.prio);
         const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
         switch (pd.prob * 2 + (cmp ? 1 : 0)) {
         case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;

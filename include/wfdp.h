@@ -71,7 +71,10 @@ typedef struct {
 #define WF_TILE_ED_TEXTBOOK 0x1 /* ED row 0 = x (reference GPU_setInit), not ED.cpp's x-1 */
 #define WF_TILE_CACHE 0x2       /* hyperlane-Cache policy: one wave per workgroup, all
                                    strip boundaries through L2 (GPU_Hyperlane_Cache.cu) */
-#define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine (the default) */
+/* Engine choice; without either flag the planner takes the asynchronous
+ * engine when all strips of the launch fit on the chip at once (at most two
+ * 512-row strips per CU) and the lock-step engine otherwise. */
+#define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine */
 #define WF_TILE_ASYNC 0x8       /* asynchronous engine: LDS progress counters + I/O wave */
 
 typedef struct {
