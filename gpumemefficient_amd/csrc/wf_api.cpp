@@ -62,9 +62,22 @@ int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, 
     const int scale = (problem == WF_SW) ? 4 : 1;  // SW codes are bit offsets of 4-bit fields
     *mode = 0;
     if (n1 + n2 == 0) return WF_OK;
+    // Unmasked steps left of column 0 (WFK_MODE_FREE_HEAD): always for LCS
+    // and ED; for SW only with a code that matches no row -- the unused last
+    // 4-bit field of the row tables, or any value above the codes in CMP mode.
+    auto free_head = [&](int max_symbol, bool cmp) {
+        if (problem != WF_SW) {
+            *mode |= WFK_MODE_FREE_HEAD;
+        } else if (cmp) {
+            *mode |= WFK_MODE_FREE_HEAD | (0x1FF << WFK_MODE_NEVER_SHIFT);
+        } else if (max_symbol < table_limit - 1) {
+            *mode |= WFK_MODE_FREE_HEAD | (((table_limit - 1) * scale) << WFK_MODE_NEVER_SHIFT);
+        }
+    };
     if (lo >= 0 && hi < table_limit) {
         for (int64_t i = 0; i < n1; ++i) out_a[i] = (uint8_t)(a[i] * scale);
         for (int64_t i = 0; i < n2; ++i) out_b[i] = (uint8_t)(b[i] * scale);
+        free_head(hi, false);
         return WF_OK;
     }
     // Dense renaming in order of first appearance.
@@ -85,10 +98,12 @@ int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, 
     if (nsym <= table_limit) {
         for (int64_t i = 0; i < n1; ++i) out_a[i] = (uint8_t)(ia[i] * scale);
         for (int64_t i = 0; i < n2; ++i) out_b[i] = (uint8_t)(ib[i] * scale);
+        free_head(nsym - 1, false);
     } else {
         *mode |= WFK_MODE_CMP;
         for (int64_t i = 0; i < n1; ++i) out_a[i] = (uint8_t)ia[i];
         for (int64_t i = 0; i < n2; ++i) out_b[i] = (uint8_t)ib[i];
+        free_head(nsym - 1, true);
     }
     return WF_OK;
 }

@@ -39,8 +39,17 @@ struct ChunkGeom {
 template <int P, bool CMP>
 struct Cell;
 
+// Cell::idle: the starting state of a lane that runs unmasked steps left of
+// column 0.  Every such step maps values at columns <= -2 to values that
+// stay far from the table's (min for ED, max for LCS), and column -1 to the
+// left boundary 0 exactly, because the lane above delivers 0 there.  SW
+// starts at 0 and relies on a never-matching column code instead (its
+// saturation at 0 keeps no sentinel).
+constexpr int kIdleBig = 1 << 29;
+
 template <bool CMP>
 struct Cell<WFK_LCS, CMP> {
+    static constexpr int idle = -kIdleBig;
     static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
     {
         if (CMP) return valid ? bc : 0x100u;
@@ -55,6 +64,7 @@ struct Cell<WFK_LCS, CMP> {
 
 template <bool CMP>
 struct Cell<WFK_ED, CMP> {
+    static constexpr int idle = kIdleBig;
     static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
     {
         if (CMP) return valid ? bc : 0x100u;
@@ -69,6 +79,7 @@ struct Cell<WFK_ED, CMP> {
 
 template <bool CMP>
 struct Cell<WFK_SW, CMP> {
+    static constexpr int idle = 0;
     // BFE: 4-bit signed field at bit 4*code holds s+2: 5 on a match, -1 else.
     static __device__ __forceinline__ uint32_t row_entry(uint32_t bc, bool valid)
     {

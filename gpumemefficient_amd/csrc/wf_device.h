@@ -11,7 +11,13 @@ enum { WFK_LCS = 0, WFK_ED = 1, WFK_SW = 2 };
 enum {
     WFK_MODE_CMP = 1,       // symbol codes are compared (alphabet too large for a row table)
     WFK_MODE_TEXTBOOK = 2,  // ED: D(0,x) = x instead of ED.cpp's x-1
+    // Lanes still left of column 0 may run the unmasked step: their state
+    // starts at a value the recurrence maps back to the left boundary (LCS,
+    // ED), or (SW) the column codes left of column 0 are a code that matches
+    // no row, bits WFK_MODE_NEVER_SHIFT.. of the mode.
+    WFK_MODE_FREE_HEAD = 4,
 };
+enum { WFK_MODE_NEVER_SHIFT = 8 };
 
 // One sequence pair as the kernels see it.  x runs over a (n1 columns),
 // y over b (n2 rows).  Codes are one byte per symbol in the code arena.

@@ -45,8 +45,9 @@ __device__ TraceRec g_trace[kTraceTasks * kTraceWaves];
 
 // Steps per iteration of the steady compute loop (a multiple of the check
 // granularity G): one loop back-edge per kU steps, the counter checks in
-// between are not-taken branches.
+// between are not-taken branches.  kB: steps per block of LDS operands.
 constexpr int kU = 32;
+constexpr int kB = 8;
 
 template <int W, int G, int AW, int RS>
 struct AsyncShared {
@@ -60,7 +61,7 @@ struct AsyncShared {
     // Column codes, one per dword (a lane's 4 codes of a block are then 4
     // aligned dword reads, which LDS serves faster than unaligned or byte
     // reads): column x at x mod AW, first G slots mirrored at AW.
-    uint32_t aw[AW + kU + 4];
+    uint32_t aw[AW + kU + kB];
     // Progress counters, 64 copies each so a wave stores one value with an
     // unmasked ds_write:
     //   cnt[0]     columns staged in ring[0] and aw
@@ -90,9 +91,6 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 #ifndef WFK_LDS_HW_FENCE
 #define WFK_LDS_HW_FENCE 0
 #endif
-#ifndef WFK_IO_PRIO
-#define WFK_IO_PRIO -1
-#endif
 __device__ __forceinline__ void lds_release()
 {
 #if WFK_LDS_HW_FENCE
@@ -110,90 +108,107 @@ __device__ __forceinline__ void lds_acquire()
 #endif
 }
 
-// Operands of one 4-step block: lane 0's top neighbours of 4 columns (one
-// broadcast ds_read_b128) and the lane's 4 column codes.
-struct Block4 {
-    int4 inj;
-    uint32_t ca[4];
+// Operands of one block of kB steps: lane 0's top neighbours of kB columns
+// (broadcast ds_read_b128s) and the lane's kB column codes.  Block k+1 is
+// read while block k is computed, kB steps ahead of its use, which covers
+// the LDS latency with six waves of a workgroup sharing the LDS.
+struct Block {
+    int4 inj[kB / 4];
+    uint32_t ca[kB];
+    __device__ int top(int u) const
+    {
+        const int4 v = inj[u / 4];
+        return (u & 3) == 0 ? v.x : (u & 3) == 1 ? v.y : (u & 3) == 2 ? v.z : v.w;
+    }
 };
 
+// Read the block of steps [s, s + kB) (s a multiple of kB; RS and AW are
+// multiples of kB and the code window is mirrored past AW, so nothing wraps
+// inside a block).
 template <int AW, int RS>
-__device__ __forceinline__ void load_block(Block4 &b, const int *ring_prev, const uint32_t *aw, int s, int lane)
+__device__ __forceinline__ void load_block(Block &b, const int *ring_prev, const uint32_t *aw, int s, int lane)
 {
-    b.inj = *reinterpret_cast<const int4 *>(ring_prev + ((s + 64) & (RS - 1)));
-    const uint32_t *cp = aw + ((s - lane) & (AW - 1));
+    const int *ip = ring_prev + ((s + 64) & (RS - 1));
 #pragma unroll
-    for (int u = 0; u < 4; ++u) b.ca[u] = cp[u];
+    for (int i = 0; i < kB / 4; ++i) b.inj[i] = *reinterpret_cast<const int4 *>(ip + 4 * i);
+    const uint32_t *cp = aw + ((s - lane + 64) & (AW - 1));  // column x in slot (x + 64) mod AW
+#pragma unroll
+    for (int u = 0; u < kB; ++u) b.ca[u] = cp[u];
 }
 
-// U steps of one compute wave, starting at step q0*G (U a multiple of G; the
-// progress counters work in groups of G steps).  `blk` holds the first
-// block (4 steps) on entry and the block after the last on return.  The LDS
-// reads of block k+1 are issued before block k is computed.  At the last
+// U steps of one compute wave, starting at step q0*G (U a multiple of G, G
+// of kB; the progress counters work in groups of G steps).  `blk` holds the
+// first block on entry and the block after the last on return.  At the last
 // block of each group the counters are checked for the next group: its
 // inputs must be there (producer) and its output slots free (consumer).
-// The counters checked were loaded at the previous group's last block
-// (`vin`/`vout`, reloaded right after the check), so the check does not wait
-// on a load, and the reads after it see everything the counters vouch for
-// (LDS executes one wave's accesses in order).  A short counter sends the
+// The counters checked were loaded half a block earlier, in the previous
+// group's last block (`vin`/`vout`), so the check does not wait on a load,
+// and the reads after it see everything the counters vouch for (LDS
+// executes one wave's accesses in order).  A short counter sends the
 // wave to `wait_next`, out of line.  The group's progress is published after
 // its last block.  There is one load site per block, so the loop-carried
 // block stays in the same VGPRs.  Returns false if a wait was abandoned.
 template <int P, bool CMP, int R, int G, int U, int AW, int RS, bool MASKED, bool TRACK, class NeedIn,
           class NeedOut, class WaitFn, class PubFn>
-__device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int lane, int n1, const int *ring_prev,
-                                          int *ring_cur, int *dummy, const uint32_t *aw, Block4 &blk,
+__device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int lane, int n1, int xlo,
+                                          const int *ring_prev,
+                                          int *ring_cur, int *dummy, const uint32_t *aw, Block &blk,
                                           const int *cin, const int *cout, int &vin, int &vout,
                                           NeedIn &&need_in, NeedOut &&need_out, WaitFn &&wait_next,
                                           PubFn &&publish)
 {
-    static_assert(U % G == 0 && G % 4 == 0, "bad unroll");
+    static_assert(U % G == 0 && G % kB == 0, "bad unroll");
     const int s0 = q0 * G;
     // s0 is a multiple of U, so neither the ring nor the (mirrored) code
     // window wraps inside the iteration; only the next iteration's first
     // block is addressed afresh.
     const int *__restrict__ ib = ring_prev + ((s0 + 64) & (RS - 1));
-    const uint32_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
+    const uint32_t *__restrict__ cp = aw + ((s0 - lane + 64) & (AW - 1));
     int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
     int *cap_last = (lane == 63) ? ring_cur + ((s0 + U) & (RS - 1)) : dummy + lane + (U - 1);
     bool ok = true;
 #pragma unroll
-    for (int q = 0; q < U; q += 4) {
-        const int injv[4] = {blk.inj.x, blk.inj.y, blk.inj.z, blk.inj.w};
-        const uint32_t cav[4] = {blk.ca[0], blk.ca[1], blk.ca[2], blk.ca[3]};
+    for (int q = 0; q < U; q += kB) {
+        int injv[kB];
+        uint32_t cav[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            injv[u] = blk.top(u);
+            cav[u] = blk.ca[u];
+        }
         const int gi = q0 + q / G;  // group of this block
-        if (q % G == G - 4) {
+        if (q % G == G - kB) {
             // Wave-uniform: every lane reads its own copy of the counters.
             const int slack = min(__builtin_amdgcn_readfirstlane(vin) - need_in(gi + 1),
                                   __builtin_amdgcn_readfirstlane(vout) - need_out(gi + 1));
             if (__builtin_expect(slack < 0 && gi + 1 < ngr, 0)) ok = ok && wait_next(gi + 1);
             lds_acquire();
-            vin = cnt_load(cin);
-            vout = cnt_load(cout);
         }
-        if (q + 4 < U) {
-            blk.inj = *reinterpret_cast<const int4 *>(ib + q + 4);
+        // Past the last group the next block reads stale slots (in bounds)
+        // that are never used.
+        const int *ip = (q + kB < U) ? ib + q + kB : ring_prev + ((s0 + U + 64) & (RS - 1));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) blk.ca[u] = cp[q + 4 + u];
-        } else {
-            // Past the last group this reads stale slots (in bounds) that
-            // are never used.
-            blk.inj = *reinterpret_cast<const int4 *>(ring_prev + ((s0 + U + 64) & (RS - 1)));
+        for (int i = 0; i < kB / 4; ++i) blk.inj[i] = *reinterpret_cast<const int4 *>(ip + 4 * i);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) blk.ca[u] = cp[U + u];
-        }
+        for (int u = 0; u < kB; ++u) blk.ca[u] = cp[q + kB + u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kB; ++u) {
             const int j = q + u;
             bool valid = true;
-            if (MASKED) valid = (unsigned)(s0 + j - lane) < (unsigned)n1;
+            // Columns [xlo, n1) are computed (xlo = -64 with a free head).
+            if (MASKED) valid = (unsigned)(s0 + j - lane - xlo) < (unsigned)(n1 - xlo);
             step<P, CMP, R, MASKED, TRACK>(st, injv[u], cav[u], valid);
             if (j == U - 1)
                 *cap_last = st.h[R - 1];
             else
                 cap[j] = st.h[R - 1];
+            // Counters for the next check, half a block before it.
+            if (q % G == G - kB && u == kB / 2 - 1) {
+                vin = cnt_load(cin);
+                vout = cnt_load(cout);
+            }
         }
-        if (q % G == G - 4) publish(gi);
+        if (q % G == G - kB) publish(gi);
     }
     return ok;
 }
@@ -252,6 +267,14 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         }
         st.dg = 0;
         st.best = 0;
+        // Lanes left of column 0 run unmasked steps (see Cell::idle): lane 0
+        // starts at column 0, lane l >= 1 at column -l from the idle state.
+        const bool free_head = (pd.mode & WFK_MODE_FREE_HEAD) != 0;
+        if (free_head && lane > 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) st.h[r] = Cell<P, CMP>::idle;
+            st.dg = Cell<P, CMP>::idle;
+        }
 
         const int *cin = sh.cnt[w] + lane;        // producer of my input
         const int *cout = sh.cnt[w + 2] + lane;   // consumer of my output ring
@@ -283,9 +306,11 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                     __device__ ~Acc() { *acc += clock64() - c0; }
                 } acc{c0, &tr_wait};
 #endif
+                // Spin first (the producer is usually a few steps away; an
+                // LDS round trip is the natural poll period), then sleep.
                 const int64_t t0 = now_ticks();
-                for (;;) {
-                    __builtin_amdgcn_s_sleep(1);
+                for (int spins = 0;; ++spins) {
+                    if (spins >= 64) __builtin_amdgcn_s_sleep(1);
                     if (ready(q)) break;
                     if (__builtin_amdgcn_readfirstlane(
                             __hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ||
@@ -296,7 +321,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
             lds_acquire();
             return true;
         };
-        Block4 blk;
+        Block blk;
         bool aborted = !wait_ready(0);
 #ifdef WFK_TRACE
         tr_start = __builtin_amdgcn_s_memrealtime();
@@ -305,9 +330,16 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
 #endif
         load_block<AW, RS>(blk, sh.ring[w], sh.aw, 0, lane);
         int vin = cnt_load(cin), vout = cnt_load(cout);
+        // Published progress, kept in a VGPR of its own (it is loaded, so the
+        // compiler treats it as per-lane): materialising it from an SGPR right
+        // before the store would reuse a register of a capture store still in
+        // flight and make the wave wait for its block reads.
+        int prog = cnt_load(cmine);  // 0: counters are cleared before each strip
         auto publish = [&](int gi) {
+            (void)gi;
+            prog = min(nsteps, prog + G);
             lds_release();
-            cnt_store(cmine, min(nsteps, (gi + 1) * G));
+            cnt_store(cmine, prog);
         };
         auto wait_next = [&](int gi) { return wait_ready(gi); };
         // Steps whose columns are valid for every lane run the unmasked body,
@@ -315,19 +347,22 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         // 0 at the start, past column n1 - 1 at the end) run one group per
         // iteration.  One body per loop keeps `blk` in fixed registers.
         constexpr int UG = kU / G;
-        auto steady_from = [&](int q) { return q * G >= 63 && q * G + kU - 1 <= n1 - 1 && q + UG <= ngr; };
+        const int xlo = free_head ? -64 : 0;
+        auto steady_from = [&](int q) {
+            return (free_head || q * G >= 63) && q * G + kU - 1 <= n1 - 1 && q + UG <= ngr;
+        };
         int q = 0;
         for (; !aborted && q < ngr && !(steady_from(q) && (q * G) % kU == 0); ++q)
             aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
-                st, q, ngr, lane, n1, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
+                st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
                 need_in, need_out, wait_next, publish);
         for (; !aborted && steady_from(q); q += UG)
             aborted = !run_steps<P, CMP, R, G, kU, AW, RS, false, TRACK>(
-                st, q, ngr, lane, n1, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
+                st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
                 need_in, need_out, wait_next, publish);
         for (; !aborted && q < ngr; ++q)
             aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
-                st, q, ngr, lane, n1, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
+                st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
                 need_in, need_out, wait_next, publish);
         if (aborted) {
             if (lane == 0) __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -355,128 +390,155 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         return;
     }
 
-    // -------------------------------------------------------------- I/O wave
-#if WFK_IO_PRIO >= 0
-    // Below the compute waves: the I/O wave shares a SIMD with one of them
-    // and should only take issue slots that wave leaves idle.
-    __builtin_amdgcn_s_setprio(WFK_IO_PRIO);
-#endif
+    // The two I/O waves share SIMDs with compute waves 0 and 1 and run below
+    // them: they only take issue slots those waves leave idle.
+    __builtin_amdgcn_s_setprio(0);
     const uint8_t *acodes = ka.codes + pd.a_off;
     const uint64_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
     uint64_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
+    const int *clast = sh.cnt[wa] + lane;   // last compute wave: slowest reader of aw, producer of ring[W]
+    bool aborted = false;
+
+    if (wave == W + 1) {
+        // ------------------------------------------------------ output wave
+        // Stores the strip's bottom row (ring[W], written by lane 63 of the
+        // last compute wave) to HBM as soon as 8 columns are finished, and
+        // publishes the stored count in cnt[W+1] (which frees ring slots).
+        if (!has_next) {
+            // No consumer of the bottom row: never block the last wave.
+            cnt_store(sh.cnt[W + 1] + lane, 1 << 30);
+        } else {
+            int out_done = 0;
+            int64_t t_idle = now_ticks();
+            while (out_done < n1) {
+                const int pl = __builtin_amdgcn_readfirstlane(cnt_load(clast));
+                const int avail = min(n1, pl - 63);
+                if (avail - out_done >= 8 || (avail == n1 && avail > out_done)) {
+                    const int end = min(avail, out_done + 64);
+                    const int x = out_done + lane;
+                    lds_acquire();
+                    if (x < end) {
+                        const int v = sh.ring[W][(x + 64) & (RS - 1)];
+                        __hip_atomic_store(bnext + x, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    out_done = end;
+                    lds_release();
+                    cnt_store(sh.cnt[W + 1] + lane, out_done);
+                    t_idle = now_ticks();
+                    continue;
+                }
+                if (__builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ||
+                    now_ticks() - t_idle > ka.timeout_ticks) {
+                    aborted = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (aborted) {
+                // Mark the pair and release the strip below with placeholder
+                // values so that it does not wait out its own timeout in turn.
+                if (lane == 0) {
+                    atomicOr(&ka.res[pair].status, 1);
+                    __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                for (int x = out_done + lane; x < n1; x += 64)
+                    __hip_atomic_store(bnext + x, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+#ifdef WFK_TRACE
+        tr_store();
+#endif
+        return;
+    }
+
+    // ---------------------------------------------------------- input wave
+    // Stages the strip's top boundary (the previous strip's bottom row, as
+    // it appears in HBM) and the column codes into LDS and publishes the
+    // staged count in cnt[0].  Two polls of the next 64 unstaged columns are
+    // kept in flight, half a poll latency apart, so a column is seen about a
+    // quarter latency after it lands instead of half a latency; each poll
+    // stages its ready prefix (the lanes up to the first stale tag).
     // Row 0 of the table in the kernels' value representation.
     const int topv = (P == WFK_ED && !(pd.mode & WFK_MODE_TEXTBOOK)) ? -1 : 0;
     const int *c0 = sh.cnt[1] + lane;       // compute wave 0: reads ring[0] and aw
-    const int *clast = sh.cnt[wa] + lane;   // last compute wave: slowest reader of aw, producer of ring[W]
-
-    int in_next = 0;    // columns staged and published in cnt[0]
-    int out_done = 0;   // bottom-row columns stored and published in cnt[W+1]
-    uint64_t top[2];
-    uint32_t code[2];
-    int have = 0;       // batches loaded: columns [in_next, in_next + 64*have)
-    int staged = 0;     // leading columns of the current batch already staged
-
-    auto load_batch = [&](int base, uint64_t &t, uint32_t &c) {
-        const int x = base + lane;
-        const bool in = x < n1;
-        c = in ? (uint32_t)acodes[x] : 0u;
-        t = (has_prev && in) ? __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tag;
+    // Column x lives in code slot (x + 64) mod AW; the slots of columns
+    // -64..-1, read by lanes left of column 0, hold a code that matches no
+    // row (see WFK_MODE_FREE_HEAD).  Written before the first staging
+    // counter store, so every compute wave reads them after they are set.
+    {
+        const uint32_t never = ((uint32_t)pd.mode >> WFK_MODE_NEVER_SHIFT) & 0x1FFu;
+        sh.aw[lane] = never;
+        if (lane < kU + kB) sh.aw[AW + lane] = never;
+    }
+    int in_next = 0;  // columns staged and published in cnt[0]
+    struct Poll {
+        uint64_t t;
+        uint32_t c;
+        int base;
     };
-
-    // The last strip has no consumer of its bottom row: never block wave W-1.
-    if (!has_next) cnt_store(sh.cnt[W + 1] + lane, 1 << 30);
-
+    auto issue = [&](Poll &p) {
+        p.base = in_next;
+        const int x = in_next + lane;
+        const bool in = x < n1;
+        p.c = in ? (uint32_t)acodes[x] : 0u;
+        p.t = (has_prev && in) ? __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tag;
+    };
+    // Stage what poll p shows beyond in_next, within the room the compute
+    // waves have freed: ring[0] slot of column c is free once wave 0 has
+    // read it (steps < p0 done), its code slot once the last wave has
+    // (it reads columns >= pl - 63).  Then poll again from in_next.
+    auto examine = [&](Poll &p, int p0, int pl) -> bool {
+        const bool stale = (p.t >> 32) != (uint64_t)epoch;
+        const uint64_t smask = __builtin_amdgcn_ballot_w64(stale);
+        const int ready = smask ? (int)__builtin_ctzll(smask) : 64;
+        const int room = min(p0 + RS, pl - 63 - G + AW);
+        const int limit = min(min(n1, p.base + ready), room);
+        bool did = false;
+        if (limit > in_next) {
+            const int x = p.base + lane;
+            if (x >= in_next && x < limit) {
+                sh.ring[0][(x + 64) & (RS - 1)] = has_prev ? (int)(uint32_t)p.t : topv;
+                const int slot = (x + 64) & (AW - 1);
+                sh.aw[slot] = p.c;
+                if (slot < kU + kB) sh.aw[AW + slot] = p.c;
+            }
+            in_next = limit;
+            lds_release();
+            cnt_store(sh.cnt[0] + lane, in_next);
+            did = true;
+        }
+        if (in_next < n1) issue(p);
+        return did;
+    };
+    Poll pa, pb;
+    issue(pa);
+    if (has_prev) __builtin_amdgcn_s_sleep(16);  // about half a poll latency
+    issue(pb);
     // Idle time only counts while the compute waves are not moving either,
-    // i.e. while everything waits on the producing strip.  The wave sleeps
-    // whenever it moved no data, so it takes few issue slots from the compute
-    // wave that shares its SIMD.
+    // i.e. while everything waits on the producing strip.
     int64_t t_idle = now_ticks();
     int seen = -1;
-    bool aborted = false;
-    for (;;) {
-        bool did = false;
-        const int p0 = cnt_load(c0), pl = cnt_load(clast);
+    while (in_next < n1) {
+        int p0 = __builtin_amdgcn_readfirstlane(cnt_load(c0));
+        int pl = __builtin_amdgcn_readfirstlane(cnt_load(clast));
+        bool did = examine(pa, p0, pl);
+        if (in_next >= n1) break;
+        did = examine(pb, p0, pl) || did;
         const bool moving = (p0 + pl != seen);
         seen = p0 + pl;
-        if (in_next < n1) {
-            // Room for column in_next + 64*k + 63: its ring[0] slot must have
-            // been read by wave 0 (steps < p0 done) and its code-window slot
-            // by wave W-1 (reads columns >= pl - 63).
-            auto room = [&](int k) {
-                const int hi = in_next + 64 * k + 63;
-                return hi - RS < p0 && hi - AW < pl - 63 - G;
-            };
-            while (have < 2 && in_next + 64 * have < n1 && room(have)) {
-                load_batch(in_next + 64 * have, top[have], code[have]);
-                ++have;
-            }
-            if (have > 0) {
-                const bool stale = (top[0] >> 32) != (uint64_t)epoch;
-                const uint64_t smask = __builtin_amdgcn_ballot_w64(stale);
-                // Columns [in_next, in_next + ready) of the batch are valid.
-                const int ready = smask ? (int)__builtin_ctzll(smask) : 64;
-                if (ready > staged) {
-                    const int x = in_next + lane;
-                    if (lane >= staged && lane < ready && x < n1) {
-                        sh.ring[0][(x + 64) & (RS - 1)] = has_prev ? (int)(uint32_t)top[0] : topv;
-                        const int slot = x & (AW - 1);
-                        sh.aw[slot] = code[0];
-                        if (slot < kU + 4) sh.aw[AW + slot] = code[0];
-                    }
-                    staged = ready;
-                    lds_release();
-                    cnt_store(sh.cnt[0] + lane, min(n1, in_next + ready));
-                    did = true;
-                }
-                if (staged == 64 || in_next + staged >= n1) {
-                    in_next = min(n1, in_next + 64);
-                    staged = 0;
-                    top[0] = top[1];
-                    code[0] = code[1];
-                    --have;
-                } else if (stale) {
-                    // The producing strip has not stored these yet: re-poll.
-                    top[0] = __hip_atomic_load(bprev + in_next + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-        }
-        if (has_next && out_done < n1) {
-            const int avail = min(n1, pl - 63);
-            if (avail > out_done && (avail - out_done >= 8 || avail == n1)) {
-                const int end = min(avail, out_done + 64);
-                const int x = out_done + lane;
-                if (x < end) {
-                    const int v = sh.ring[W][(x + 64) & (RS - 1)];
-                    __hip_atomic_store(bnext + x, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                out_done = end;
-                lds_release();
-                cnt_store(sh.cnt[W + 1] + lane, out_done);
-                did = true;
-            }
-        }
-        if (in_next >= n1 && (!has_next || out_done >= n1)) break;
-        if (__hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        if (did || moving) t_idle = now_ticks();
+        if (__builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ||
+            now_ticks() - t_idle > ka.timeout_ticks) {
             aborted = true;
             break;
         }
-        if (did || moving) t_idle = now_ticks();
-        if (!did) {
-            __builtin_amdgcn_s_sleep(2);
-            if (now_ticks() - t_idle > ka.timeout_ticks) {
-                aborted = true;
-                if (lane == 0) __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                break;
-            }
-        }
+        if (!did && !has_prev) __builtin_amdgcn_s_sleep(2);  // only waiting for room
     }
-    if (aborted) {
-        // Mark the pair and release the strip below with placeholder values
-        // so that it does not wait out its own timeout in turn.
-        if (lane == 0) atomicOr(&ka.res[pair].status, 1);
-        if (has_next)
-            for (int x = out_done + lane; x < n1; x += 64)
-                __hip_atomic_store(bnext + x, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (aborted && lane == 0) {
+        atomicOr(&ka.res[pair].status, 1);
+        __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 #ifdef WFK_TRACE
     tr_store();
@@ -491,7 +553,7 @@ struct AsyncCfg {
 };
 
 template <int R, int W, int G>
-__global__ __launch_bounds__(64 * (W + 1)) void wf_engine_async(WfKernelArgs ka)
+__global__ __launch_bounds__(64 * (W + 2)) void wf_engine_async(WfKernelArgs ka)
 {
     constexpr int AW = AsyncCfg<R, W, G>::AW;
     constexpr int RS = AsyncCfg<R, W, G>::RS;
@@ -526,7 +588,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void wf_engine_async(WfKernelArgs ka)
 }  // namespace wfk
 
 #define WF_ASYNC(R_, W_, G_) \
-    {WF_ENGINE_ASYNC, R_, W_, G_, 64 * ((W_) + 1), reinterpret_cast<const void *>(&wfk::wf_engine_async<R_, W_, G_>)}
+    {WF_ENGINE_ASYNC, R_, W_, G_, 64 * ((W_) + 2), reinterpret_cast<const void *>(&wfk::wf_engine_async<R_, W_, G_>)}
 
 extern "C" const WfVariant wf_variants_async[] = {
     WF_ASYNC(1, 4, 16), WF_ASYNC(2, 4, 16), WF_ASYNC(4, 4, 16),
