@@ -51,22 +51,25 @@ constexpr int kB = 8;
 
 template <int W, int G, int AW, int RS>
 struct AsyncShared {
-    // ring[0]: the strip's top input, staged by the I/O wave; ring[w+1]:
-    // lane 63 of compute wave w, read by wave w+1 (w = W-1: by the I/O wave).
-    // Column x lives in slot (x + 64) mod RS.
+    // Column codes, one per dword (a lane's codes of a block are then aligned
+    // dword reads, which LDS serves faster than unaligned or byte reads):
+    // column x at slot (x + 64) mod AW, the first kU + kB slots mirrored at
+    // AW so that a lane's reads of one loop iteration never wrap.  First in
+    // the workgroup's LDS so that a lane's reads fit the 8-bit dword offsets
+    // of ds_read2 without extra address arithmetic.
+    __attribute__((aligned(16))) uint32_t aw[AW + kU + kB];
+    // ring[0]: the strip's top input, staged by the input wave; ring[w+1]:
+    // lane 63 of compute wave w, read by wave w+1 (w = W-1: by the output
+    // wave).  Column x lives in slot (x + 64) mod RS.
     __attribute__((aligned(16))) int ring[W + 1][RS];
     // Lanes 0..62 store their bottom cell here so the capture is one
     // unmasked ds_write per step.
     int dummy[W][64 + kU];
-    // Column codes, one per dword (a lane's 4 codes of a block are then 4
-    // aligned dword reads, which LDS serves faster than unaligned or byte
-    // reads): column x at x mod AW, first G slots mirrored at AW.
-    uint32_t aw[AW + kU + kB];
     // Progress counters, 64 copies each so a wave stores one value with an
     // unmasked ds_write:
-    //   cnt[0]     columns staged in ring[0] and aw
+    //   cnt[0]     columns staged in ring[0] and aw (input wave)
     //   cnt[1+w]   steps finished by compute wave w
-    //   cnt[W+1]   bottom-row columns stored to HBM by the I/O wave
+    //   cnt[W+1]   bottom-row columns stored to HBM (output wave)
     int cnt[W + 2][64];
     int task;
     int abort;

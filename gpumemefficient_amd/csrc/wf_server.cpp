@@ -133,9 +133,20 @@ static int launch_slot(wf_server *s, Slot &sl, std::vector<Request> &&batch)
     sl.ph = WfPlanHost();
     int st = wf_build_plan(s->ctx, jobs.data(), (int64_t)jobs.size(), nullptr, &sl.ph);
     if (st != WF_OK) return st;
-    if ((st = wf_upload_plan_async(s->ctx, &sl.ph, &sl.ar, sl.stream, false)) != WF_OK) return st;
     const int nslots = (int)s->slots.size();
-    const int maxg = std::max(1, wf_max_resident_groups(s->ctx, sl.ph.variant) / std::max(1, nslots));
+    auto share = [&](const WfVariant *v) {
+        return std::max(1, wf_max_resident_groups(s->ctx, v) / std::max(1, nslots));
+    };
+    // The asynchronous engine only pays off when all strips of the launch are
+    // resident within the slot's share of the chip; otherwise the lock-step
+    // engine's higher occupancy wins.
+    if (sl.ph.variant->engine == WF_ENGINE_ASYNC && (int64_t)sl.ph.tasks.size() > share(sl.ph.variant)) {
+        const wf_tile lock{0, 0, 0, WF_TILE_LOCKSTEP};
+        sl.ph = WfPlanHost();
+        if ((st = wf_build_plan(s->ctx, jobs.data(), (int64_t)jobs.size(), &lock, &sl.ph)) != WF_OK) return st;
+    }
+    if ((st = wf_upload_plan_async(s->ctx, &sl.ph, &sl.ar, sl.stream, false)) != WF_OK) return st;
+    const int maxg = share(sl.ph.variant);
     if ((st = wf_enqueue_plan(s->ctx, &sl.ph, &sl.ar, WF_RUN_PERSISTENT, maxg, sl.stream)) != WF_OK) return st;
     const int64_t n = sl.ph.npairs;
     if (n > sl.pinned_cap) {
