@@ -173,12 +173,19 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
     // strips than CUs is bound by throughput, where the lock-step engine's
     // higher occupancy wins (measured on MI355X: SW 131072^2 and ED 65536^2
     // against the 4096-pair batch).
+    // The async engine's counter group G trades per-step cost (one check per
+    // group) against the lag a wave adds to the chain (about 2G + 63 steps):
+    // with half a chip of strips or fewer the steps dominate (ED 65536^2:
+    // G = 32 3.47 ms, G = 16 3.57 ms), with more the lag does (SW 131072^2:
+    // G = 16 8.29 ms, G = 32 8.41 ms).
+    int async_k = 16;
     {
         int64_t strips = 0;
         for (int64_t i = 0; i < njobs; ++i)
             if (jobs[i].n1 > 0 && jobs[i].n2 > 0) strips += (jobs[i].n2 + 511) / 512;
         const int cus = ctx ? ctx->prop.multiProcessorCount : 256;
         if (strips <= 2LL * cus) E = WF_ENGINE_ASYNC;
+        if (strips <= cus / 2) async_k = 32;
     }
     if (tile && (tile->flags & WF_TILE_ASYNC)) E = WF_ENGINE_ASYNC;
     if (tile && (tile->flags & WF_TILE_LOCKSTEP)) E = WF_ENGINE_LOCKSTEP;
@@ -208,7 +215,7 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
     }
     if (W == 0) W = cache ? 1 : wf_default_waves(max_rows, total_cells, (int)njobs);
     if (R == 0) R = wf_default_rows_per_lane(W, max_rows, total_cells, (int)njobs);
-    if (K == 0) K = (E == WF_ENGINE_LOCKSTEP) ? 64 : 16;
+    if (K == 0) K = (E == WF_ENGINE_LOCKSTEP) ? 64 : async_k;
     // Snap to the nearest compiled variant of the requested engine.
     const WfVariant *best = nullptr;
     int best_d = 1 << 30;

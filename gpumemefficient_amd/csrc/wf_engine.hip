@@ -91,6 +91,12 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 // from moving the accesses across the counter (a signal fence).  Building
 // with -DWFK_LDS_HW_FENCE=1 uses s_waitcnt-based workgroup fences instead
 // (which drain the read pipeline at every group boundary).
+// Sleep (s_sleep units of 64 clocks) of the output wave between polls of
+// the last compute wave's progress, and of the input wave while it has no
+// room to stage into.
+#ifndef WFK_IO_SLEEP
+#define WFK_IO_SLEEP 3
+#endif
 #ifndef WFK_LDS_HW_FENCE
 #define WFK_LDS_HW_FENCE 0
 #endif
@@ -436,7 +442,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                     aborted = true;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_s_sleep(WFK_IO_SLEEP);
             }
             if (aborted) {
                 // Mark the pair and release the strip below with placeholder
@@ -537,7 +543,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
             aborted = true;
             break;
         }
-        if (!did && !has_prev) __builtin_amdgcn_s_sleep(2);  // only waiting for room
+        if (!did && !has_prev) __builtin_amdgcn_s_sleep(WFK_IO_SLEEP);  // only waiting for room
     }
     if (aborted && lane == 0) {
         atomicOr(&ka.res[pair].status, 1);
