@@ -91,11 +91,19 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 // from moving the accesses across the counter (a signal fence).  Building
 // with -DWFK_LDS_HW_FENCE=1 uses s_waitcnt-based workgroup fences instead
 // (which drain the read pipeline at every group boundary).
+// Step of a group's last block after which the counters for the next check
+// are loaded.
+#ifndef WFK_CNT_AT
+#define WFK_CNT_AT (kB / 2 - 1)
+#endif
 // Sleep (s_sleep units of 64 clocks) of the output wave between polls of
 // the last compute wave's progress, and of the input wave while it has no
 // room to stage into.
 #ifndef WFK_IO_SLEEP
 #define WFK_IO_SLEEP 3
+#endif
+#ifndef WFK_IN_SLEEP
+#define WFK_IN_SLEEP 16
 #endif
 #ifndef WFK_LDS_HW_FENCE
 #define WFK_LDS_HW_FENCE 0
@@ -211,10 +219,14 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
                 *cap_last = st.h[R - 1];
             else
                 cap[j] = st.h[R - 1];
-            // Counters for the next check, half a block before it.
-            if (q % G == G - kB && u == kB / 2 - 1) {
+            // Counters for the next check, half a block before it (the
+            // scheduling barriers keep the compiler from sinking the loads
+            // to just before their use).
+            if (q % G == G - kB && u == WFK_CNT_AT) {
+                __builtin_amdgcn_sched_barrier(0);
                 vin = cnt_load(cin);
                 vout = cnt_load(cout);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (q % G == G - kB) publish(gi);
@@ -543,7 +555,9 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
             aborted = true;
             break;
         }
-        if (!did && !has_prev) __builtin_amdgcn_s_sleep(WFK_IO_SLEEP);  // only waiting for room
+        // Only waiting for room: the compute waves free it at one column per
+        // step, and the wave shares a SIMD with compute wave 0.
+        if (!did && !has_prev) __builtin_amdgcn_s_sleep(WFK_IN_SLEEP);
     }
     if (aborted && lane == 0) {
         atomicOr(&ka.res[pair].status, 1);
