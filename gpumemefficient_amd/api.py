@@ -185,6 +185,17 @@ class Context:
                                  C.byref(out)), "wf_align")
         return Result._from(out)
 
+    def align_table(self, problem, a, b, ed_textbook: bool = False) -> tuple[Result, np.ndarray]:
+        """Score of one pair plus its whole DP table as an (n2, n1) int32
+        array, table[y-1, x-1] = H(y, x) (the reference drivers' DEBUG dump)."""
+        a, b = _as_i32(a), _as_i32(b)
+        t = _lib.wf_tile(0, 0, 0, _lib.WF_TILE_ED_TEXTBOOK if ed_textbook else 0)
+        table = np.empty((b.size, a.size), dtype=np.int32)
+        out = _lib.wf_result()
+        check(self._lib.wf_align_table(self._h, _problem_id(problem), _ptr(a), a.size, _ptr(b), b.size, C.byref(t),
+                                       _ptr(table), C.byref(out)), "wf_align_table")
+        return Result._from(out), table
+
     def align_batch(self, jobs: Iterable[Job], tile: Tile | None = None) -> list[Result]:
         ja = _JobArray(jobs)
         out = (_lib.wf_result * max(1, ja.n))()

@@ -612,6 +612,54 @@ int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32
     return wf_align_batch(ctx, &j, 1, tile, out);
 }
 
+int wf_align_table(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
+                   const wf_tile *tile, int32_t *table, wf_result *out)
+{
+    if (!ctx || !out || !table || n1 <= 0 || n2 <= 0 || n1 * n2 > WF_TABLE_MAX_CELLS) return WF_ERR_INVALID;
+    if (hipSetDevice(ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    const int flags = tile ? (tile->flags & WF_TILE_ED_TEXTBOOK) : 0;
+    const wf_job j{problem, flags, a, n1, b, n2};
+    // The table kernel is the lock-step sweep <R=2, W=4, K=32>; the plan's
+    // strips must have its geometry.
+    const wf_tile lock{32, 512, 2, WF_TILE_LOCKSTEP | flags};
+    WfPlanHost ph;
+    int st = wf_build_plan(ctx, &j, 1, &lock, &ph);
+    if (st != WF_OK) return st;
+    const WfVariant *v = ph.variant;
+    if (v->engine != WF_ENGINE_LOCKSTEP || v->rows_per_lane != 2 || v->waves != 4 || v->chunk != 32)
+        return WF_ERR_NOKERNEL;
+    if ((st = wf_upload_plan(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
+    DevBuf dt;
+    if ((st = dt.ensure((size_t)(n1 * n2) * sizeof(int32_t))) != WF_OK) return st;
+    WfArenas *ar = &ctx->scratch;
+    if (wf_launch_reset((WfResultDev *)ar->res.ptr, 1, (int32_t *)ar->counters.ptr, 2, (uint32_t *)ar->epoch.ptr,
+                        ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    WfKernelArgs ka{};
+    ka.pairs = (const WfPairDev *)ar->pairs.ptr;
+    ka.tasks = (const WfTaskDev *)ar->tasks.ptr;
+    ka.codes = (const uint8_t *)ar->codes.ptr;
+    ka.bnd = (uint64_t *)ar->bnd.ptr;
+    ka.res = (WfResultDev *)ar->res.ptr;
+    ka.epoch = (const uint32_t *)ar->epoch.ptr;
+    ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
+    ka.counter = (int32_t *)ar->counters.ptr;
+    ka.task_begin = 0;
+    ka.task_end = (int32_t)ph.tasks.size();
+    ka.table = (int32_t *)dt.ptr;
+    const int grid = (int)std::min<int64_t>((int64_t)ph.tasks.size(), ctx->prop.multiProcessorCount);
+    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (grid > 0 && wf_launch_table(grid, &ka, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (hipMemcpy(table, dt.ptr, (size_t)(n1 * n2) * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return WF_ERR_DEVICE;
+    dt.release();
+    return wf_download_results(ctx, &ph, ar, ms, out, 1);
+}
+
 int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile, wf_plan **out)
 {
     if (!ctx || !out || njobs <= 0 || !jobs) return WF_ERR_INVALID;

@@ -2,6 +2,7 @@
 // ABI.  Built twice: -DWF_COMPAT_POLICY_SHARE and -DWF_COMPAT_POLICY_CACHE.
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "wfdp.h"
 #include "wfdp_compat.h"
@@ -20,7 +21,24 @@ static void fail(int st)
     std::exit(EXIT_FAILURE);
 }
 
-static int run(int problem, int n1, int n2, const int *arr1, const int *arr2, int tX, int tY)
+// The reference copies its device table back to `table` only in DEBUG
+// builds (LCS/GPU_Hyperlane_Share.cu:770-772).  Here the environment variable
+// WFDP_DEBUG_TABLE=1 plays that role: with it, a non-null table of a square
+// pair receives the whole DP table in the reference's padded layout,
+// table[(i + paddY) * (n2 + paddX) + (j + paddX)] = H(i + 1, j + 1), which its
+// drivers' DEBUG dump prints (LCS/GPU_Hyperlane_Share.cpp:109-131).  Without
+// it the score-only path runs and `table` is left alone.
+static bool debug_table()
+{
+    static const int on = [] {
+        const char *e = std::getenv("WFDP_DEBUG_TABLE");
+        return (e && *e && *e != '0') ? 1 : 0;
+    }();
+    return on != 0;
+}
+
+static int run(int problem, int n1, int n2, const int *arr1, const int *arr2, int tX, int tY, int paddX = 0,
+               int paddY = 0, int *table = nullptr)
 {
     if (!g_ctx) {
         const int st = wf_ctx_create(0, &g_ctx);
@@ -28,20 +46,29 @@ static int run(int problem, int n1, int n2, const int *arr1, const int *arr2, in
     }
     wf_tile t{tX, tY, 0, kPolicyFlags};
     wf_result r{};
-    const int st = wf_align(g_ctx, problem, reinterpret_cast<const int32_t *>(arr1), n1,
-                            reinterpret_cast<const int32_t *>(arr2), n2, &t, &r);
+    const int32_t *a = reinterpret_cast<const int32_t *>(arr1), *b = reinterpret_cast<const int32_t *>(arr2);
+    if (table && debug_table() && n1 == n2 && n1 > 0 && (int64_t)n1 * n2 <= WF_TABLE_MAX_CELLS) {
+        std::vector<int32_t> h((size_t)n1 * n2);
+        const int st = wf_align_table(g_ctx, problem, a, n1, b, n2, &t, h.data(), &r);
+        if (st != WF_OK) fail(st);
+        for (int i = 0; i < n2; i++)
+            for (int j = 0; j < n1; j++)
+                table[(int64_t)(i + paddY) * (n2 + paddX) + (j + paddX)] = h[(size_t)i * n1 + j];
+        return r.last;
+    }
+    const int st = wf_align(g_ctx, problem, a, n1, b, n2, &t, &r);
     if (st != WF_OK) fail(st);
     return r.last;
 }
 
-int LCS(int n1, int n2, int *arr1, int *arr2, int, int, int *)
+int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table)
 {
-    return run(WF_LCS, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY);
+    return run(WF_LCS, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY, paddX, paddY, table);
 }
 
-int LCS(int n1, int n2, int *arr1, int *arr2, int, int, int *, int tX, int tY)
+int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY)
 {
-    return run(WF_LCS, n1, n2, arr1, arr2, tX, tY);
+    return run(WF_LCS, n1, n2, arr1, arr2, tX, tY, paddX, paddY, table);
 }
 
 int LCS(int n1, int n2, int *arr1, int *arr2)
@@ -49,22 +76,22 @@ int LCS(int n1, int n2, int *arr1, int *arr2)
     return run(WF_LCS, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY);
 }
 
-int SW(int n1, int n2, int *arr1, int *arr2, int, int, int *, int tX, int tY)
+int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY)
 {
-    return run(WF_SW, n1, n2, arr1, arr2, tX, tY);
+    return run(WF_SW, n1, n2, arr1, arr2, tX, tY, paddX, paddY, table);
 }
 
-int SW(int n1, int n2, int *arr1, int *arr2, int, int, int *)
+int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table)
 {
-    return run(WF_SW, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY);
+    return run(WF_SW, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY, paddX, paddY, table);
 }
 
-int ED(int n1, int n2, int *arr1, int *arr2, int, int, int *, int tX, int tY)
+int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY)
 {
-    return run(WF_ED, n1, n2, arr1, arr2, tX, tY);
+    return run(WF_ED, n1, n2, arr1, arr2, tX, tY, paddX, paddY, table);
 }
 
-int ED(int n1, int n2, int *arr1, int *arr2, int, int, int *)
+int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table)
 {
-    return run(WF_ED, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY);
+    return run(WF_ED, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY, paddX, paddY, table);
 }

@@ -60,6 +60,7 @@ struct WfKernelArgs {
     int32_t task_end;
     const uint32_t *epoch;   // launch number (device memory, bumped by the reset kernel before
                              // every launch, so graph replays get fresh tags)
+    int32_t *table;          // table-dump kernel only: H(y+1, x+1) of pair 0 at [y * n1 + x]
 };
 
 #endif

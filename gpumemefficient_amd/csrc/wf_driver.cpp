@@ -15,8 +15,12 @@
 // The input is ./Data/str1_2_<e1>_str2_2_<e2>.txt; e1 and e2 only name the
 // file, the sizes come from its header (LCS/GPU_Hyperlane_Share.cpp:14-34,
 // 65-79).  The timed region is the loop of host calls, as in the reference.
-// The reference also allocates an unused (n2+3)*(n1+1) host table (68 GB at
-// 2^17); these drivers pass a null table instead.
+// The reference also allocates an (n2+3)*(n1+1) host table (68 GB at 2^17)
+// that only its DEBUG build fills and prints; these drivers pass a null table
+// to the timed calls.  With WFDP_DEBUG_TABLE=1 in the environment they make
+// one more, untimed call with a table and write it as the reference's DEBUG
+// build does: ./Output/output_Hyperlane_<e1>.txt, first line n1, then one
+// line of n2 space-terminated values per row (LCS/GPU_Hyperlane_Share.cpp:109-131).
 #include <sys/time.h>
 
 #include <cstdlib>
@@ -139,5 +143,32 @@ int main(int argc, char **argv)
 #else
     cout << "execution time: " << ms / itr << " ms." << endl;
 #endif
+    const char *dbg = getenv("WFDP_DEBUG_TABLE");
+    if (dbg && *dbg && *dbg != '0') {
+        if (n1 != n2) {
+            cout << "ERROR: the table dump needs n1 == n2." << endl;
+            exit(EXIT_FAILURE);
+        }
+        vector<int> table((size_t)(n1 + paddY) * (n2 + paddX), 0);
+#if defined(WF_DRIVER_PROBLEM_LCS)
+        LCS(n1, n2, arr1.data(), arr2.data(), paddX, paddY, table.data());
+#elif defined(WF_DRIVER_PROBLEM_SW)
+        SW(n1, n2, arr1.data(), arr2.data(), paddX, paddY, table.data());
+#else
+        ED(n1, n2, arr1.data(), arr2.data(), paddX, paddY, table.data());
+#endif
+        ostringstream outfile;
+        outfile << "./Output/output_Hyperlane_" << nn1 << ".txt";
+        ofstream output(outfile.str().c_str());
+        if (!output) {
+            cout << "ERROR: output file cannot be opened!" << endl;
+            exit(EXIT_FAILURE);
+        }
+        output << n1 << '\n';
+        for (int i = 0; i < n1; i++) {
+            for (int j = 0; j < n2; j++) output << table[(size_t)(i + paddY) * (n2 + paddX) + (j + paddX)] << ' ';
+            output << '\n';
+        }
+    }
     return 0;
 }

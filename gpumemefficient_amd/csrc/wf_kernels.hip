@@ -66,11 +66,20 @@ struct WgShared {
     int abort_at;
 };
 
+// Table-dump destination of one lane (DUMP kernels only): the lane's first
+// row and the pair's row count; values are written in the reference's
+// representation (ED: D = E + x + y with 1-based table indices).
+struct DumpRows {
+    int32_t *table;
+    int64_t row0;
+    int n2;
+};
+
 // One chunk of K steps starting at s0 (a multiple of K) for one wave.
-template <int P, bool CMP, int R, int K, int AW, bool MASKED, bool TRACK>
+template <int P, bool CMP, int R, int K, int AW, bool MASKED, bool TRACK, bool DUMP = false>
 __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, int n1,
                                           const int *ring_prev, int *ring_cur, int *dummy,
-                                          const uint32_t *aw)
+                                          const uint32_t *aw, DumpRows dr = DumpRows{})
 {
     constexpr int RS = ChunkGeom<K>::RS;
     const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (RS - 1));
@@ -101,12 +110,23 @@ __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, in
                 *cap_last = st.h[R - 1];
             else
                 cap[j] = st.h[R - 1];
+            if (DUMP) {
+                const int x = s0 + j - lane;
+                if ((unsigned)x < (unsigned)n1) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int64_t y = dr.row0 + r;
+                        if (y < dr.n2)
+                            dr.table[y * n1 + x] = (P == WFK_ED) ? st.h[r] + x + (int)y + 2 : st.h[r];
+                    }
+                }
+            }
         }
     }
 }
 
 // One strip (`g`) of one pair, swept by the whole workgroup.
-template <int P, bool CMP, int R, int W, int K, int AW>
+template <int P, bool CMP, int R, int W, int K, int AW, bool DUMP = false>
 __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
                           WgShared<W, K, AW> &sh, int lane, int wave, uint32_t epoch)
 {
@@ -211,12 +231,13 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
         if (cj >= 0 && cj < nch && wave < wa) {
             const int s0 = cj * K;
             const bool steady = (s0 >= 63) && (s0 + K - 1 <= n1 - 1);
+            const DumpRows dr{ka.table, row0, n2};
             if (steady)
-                run_chunk<P, CMP, R, K, AW, false, TRACK>(st, s0, lane, n1, sh.ring[wave],
-                                                          sh.ring[wave + 1], sh.dummy[wave], sh.aw);
+                run_chunk<P, CMP, R, K, AW, false, TRACK, DUMP>(st, s0, lane, n1, sh.ring[wave],
+                                                                sh.ring[wave + 1], sh.dummy[wave], sh.aw, dr);
             else
-                run_chunk<P, CMP, R, K, AW, true, TRACK>(st, s0, lane, n1, sh.ring[wave],
-                                                         sh.ring[wave + 1], sh.dummy[wave], sh.aw);
+                run_chunk<P, CMP, R, K, AW, true, TRACK, DUMP>(st, s0, lane, n1, sh.ring[wave],
+                                                               sh.ring[wave + 1], sh.dummy[wave], sh.aw, dr);
             if (wave == W - 1 && has_next) {
                 // Steps < s0 + K are done: lane 63 has finished columns < s0 + K - 63.
                 copy_out(min(n1, s0 + K - 63), true);
@@ -264,12 +285,8 @@ struct EngineCfg {
     static constexpr int AW = pow2_at_least((2 + (W - 1) * D) * K + 64 + K);
 };
 
-// At least 6 waves per SIMD (at most 80 VGPRs): the lock-step engine is the
-// throughput engine (batches, served mixes), where residency wins -- the
-// 4096-pair SW batch runs 1.10 ms at 80 VGPRs against 1.27 ms at the 103 the
-// compiler takes unbounded (a few spills in the edge paths).
-template <int R, int W, int K>
-__global__ __launch_bounds__(64 * W, 6) void wf_engine(WfKernelArgs ka)
+template <int R, int W, int K, bool DUMP>
+__device__ __forceinline__ void engine_body(const WfKernelArgs &ka)
 {
     constexpr int AW = EngineCfg<R, W, K>::AW;
     __shared__ WgShared<W, K, AW> sh;
@@ -290,15 +307,32 @@ __global__ __launch_bounds__(64 * W, 6) void wf_engine(WfKernelArgs ka)
         set_wave_priority(pd.prio);
         const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
         switch (pd.prob * 2 + (cmp ? 1 : 0)) {
-        case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_SW * 2 + 0: run_group<WFK_SW, false, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        default: run_group<WFK_SW, true, R, W, K, AW>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_SW * 2 + 0: run_group<WFK_SW, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        default: run_group<WFK_SW, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         }
         __syncthreads();
     }
+}
+
+// At least 6 waves per SIMD (at most 80 VGPRs): the lock-step engine is the
+// throughput engine (batches, served mixes), where residency wins -- the
+// 4096-pair SW batch runs 1.10 ms at 80 VGPRs against 1.27 ms at the 103 the
+// compiler takes unbounded (a few spills in the edge paths).
+template <int R, int W, int K>
+__global__ __launch_bounds__(64 * W, 6) void wf_engine(WfKernelArgs ka)
+{
+    engine_body<R, W, K, false>(ka);
+}
+
+// Same sweep, also writing every cell of pair 0 to ka.table (the reference
+// drivers' DEBUG table dump; small tables only).
+__global__ __launch_bounds__(256) void wf_engine_table(WfKernelArgs ka)
+{
+    engine_body<2, 4, 32, true>(ka);
 }
 
 // Zero progress counters, results and claim counters before a launch (a
@@ -343,6 +377,13 @@ extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *c
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(wfk::wf_reset, dim3(blocks), dim3(256), 0, stream, res, nres, counters, ncounters, epoch);
     return hipGetLastError();
+}
+
+extern "C" hipError_t wf_launch_table(int grid, const WfKernelArgs *ka, hipStream_t stream)
+{
+    void *args[] = {const_cast<WfKernelArgs *>(ka)};
+    return hipLaunchKernel(reinterpret_cast<const void *>(&wfk::wf_engine_table), dim3(grid), dim3(256), args, 0,
+                           stream);
 }
 
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,

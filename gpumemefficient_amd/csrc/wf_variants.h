@@ -25,6 +25,8 @@ extern "C" const int wf_num_variants_async;
 
 extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
                                       uint32_t *epoch, hipStream_t stream);
+// Lock-step <2,4,32> sweep that also writes every cell of pair 0 to ka->table.
+extern "C" hipError_t wf_launch_table(int grid, const WfKernelArgs *ka, hipStream_t stream);
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
 

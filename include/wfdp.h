@@ -124,6 +124,17 @@ int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32
              int64_t n2, const wf_tile *tile, wf_result *out);
 
 /* Many pairs, synchronous, one persistent launch. */
+/* Full DP table of one pair (the reference drivers' DEBUG dump,
+ * LCS/GPU_Hyperlane_Share.cpp:109-131): fills table[(y-1)*n1 + (x-1)] = H(y, x)
+ * for 1 <= y <= n2, 1 <= x <= n1 in the reference's values (LCS and SW cell
+ * scores, ED distances D with ED.cpp's or, with WF_TILE_ED_TEXTBOOK in tile,
+ * the textbook row 0).  Runs the lock-step engine with a per-cell store; for
+ * cell-level parity checks on small pairs (n1 * n2 <= WF_TABLE_MAX_CELLS).
+ * `table` is a caller-owned host buffer of n1 * n2 int32. */
+#define WF_TABLE_MAX_CELLS (1LL << 28)
+int wf_align_table(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
+                   int64_t n2, const wf_tile *tile, int32_t *table, wf_result *out);
+
 int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                    wf_result *out);
 

@@ -11,8 +11,11 @@
  *          (LCS/GPU_Hyperlane_Cache.cu:254-255 and the SW/ED Cache hosts)
  *
  * Semantics kept: return value = bottom-right DP cell; `table`, `paddX` and
- * `paddY` are accepted and ignored (the reference writes the host table only
- * in DEBUG builds); on a device error the process prints the error and exits
+ * `paddY` are ignored unless WFDP_DEBUG_TABLE=1 is set in the environment, the
+ * run-time counterpart of the reference's DEBUG build, which copies the whole
+ * table back (LCS/GPU_Hyperlane_Share.cu:770-772): then a square pair's
+ * table[(i + paddY) * (n2 + paddX) + (j + paddX)] holds H(i + 1, j + 1) for
+ * i, j < n (wf_align_table); on a device error the process prints the error and exits
  * with EXIT_FAILURE, as checkGPUError does (LCS/GPU_Hyperlane_Share.cu
  * :689-694).  The device is 0, as in the reference (cudaSetDevice(0)).
  */
