@@ -19,9 +19,10 @@ One JSON line on stdout:
                host on the same pair
   extra        the other headline measurements: Smith-Waterman 131072^2
                single pair; the 4096-pair batch against the loop of one
-               launch per pair; the served Poisson stream (p50/p99 latency
-               and GCUPS per rate, continuous batching against one launch
-               per request)
+               launch per pair; the reference's naive baselines (one launch
+               per cell or tile anti-diagonal) on LCS 8192^2; the served
+               Poisson stream (p50/p99 latency and GCUPS per rate, continuous
+               batching against one launch per request)
 """
 from __future__ import annotations
 
@@ -212,11 +213,31 @@ def main() -> int:
                 "per_pair_launch_gcups": round(bcells / pk / 1e6, 1), "per_pair_launch_ms": round(pk, 3),
                 "speedup": round(pk / bk, 2), "engine": p.describe()}
             p.close()
+        # The reference's naive GPU baselines on the LCS 8192 x 8192 pair:
+        # one launch per cell anti-diagonal, one launch per tile
+        # anti-diagonal, against the strip engine on the same pair.
+        lj, _, lc = make_workload("lcs")
+        p = ctx.plan(lj)
+        _, lkms = time_plan(p, 5, 1)
+        lk = sum(lkms) / len(lkms)
+        lr = p.results()[0]
+        lcells = int(lj[0].a.size) * int(lj[0].b.size)
+        naive = {"workload": lc["workload"], "engine_ms": round(lk, 3), "engine_gcups": round(lcells / lk / 1e6, 1)}
+        for method in ("diagonal", "tile"):
+            ctx.align_naive("LCS", lj[0].a, lj[0].b, method=method)
+            nr, _ = ctx.align_naive("LCS", lj[0].a, lj[0].b, method=method)
+            assert nr.last == lr.last, (method, nr.last, lr.last)
+            naive[f"naive_{method}_ms"] = round(nr.kernel_ms, 3)
+            naive[f"naive_{method}_gcups"] = round(lcells / nr.kernel_ms / 1e6, 2)
+            naive[f"speedup_vs_{method}"] = round(nr.kernel_ms / lk, 1)
+        naive["lcs_length"] = lr.last
+        extra["vs_naive_lcs_8192"] = naive
+        p.close()
         # Served mixed workload (Poisson arrivals, 50% ED / 50% SW, mixed
         # lengths), continuous batching against one launch per request.
         from gpumemefficient_amd.serve_bench import run_rate
         serve = {"requests_per_rate": args.serve_requests, "continuous": [], "per_request": []}
-        for rate in (1000.0, 10000.0, 50000.0):
+        for rate in (1000.0, 2000.0, 5000.0, 10000.0, 20000.0, 50000.0):
             serve["continuous"].append(run_rate(rate, args.serve_requests, False))
         serve["per_request"].append(run_rate(50000.0, args.serve_requests, True))
         sat_c = serve["continuous"][-1]["gcups"]

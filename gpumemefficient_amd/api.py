@@ -196,6 +196,21 @@ class Context:
                                        _ptr(table), C.byref(out)), "wf_align_table")
         return Result._from(out), table
 
+    def align_naive(self, problem, a, b, method: str = "diagonal", ed_textbook: bool = False,
+                    want_table: bool = False) -> tuple[Result, np.ndarray | None]:
+        """The reference's naive baselines: "diagonal" (one launch per cell
+        anti-diagonal) or "tile" (one launch per anti-diagonal of 64 x 128
+        tiles).  With want_table the (n2, n1) table comes back as well."""
+        a, b = _as_i32(a), _as_i32(b)
+        m = {"diagonal": _lib.WF_NAIVE_DIAGONAL, "tile": _lib.WF_NAIVE_TILE}[method]
+        t = _lib.wf_tile(0, 0, 0, _lib.WF_TILE_ED_TEXTBOOK if ed_textbook else 0)
+        table = np.empty((b.size, a.size), dtype=np.int32) if want_table else None
+        out = _lib.wf_result()
+        check(self._lib.wf_align_naive(self._h, _problem_id(problem), _ptr(a), a.size, _ptr(b), b.size, C.byref(t),
+                                       m, _ptr(table) if table is not None else None, C.byref(out)),
+              "wf_align_naive")
+        return Result._from(out), table
+
     def align_batch(self, jobs: Iterable[Job], tile: Tile | None = None) -> list[Result]:
         ja = _JobArray(jobs)
         out = (_lib.wf_result * max(1, ja.n))()

@@ -42,6 +42,7 @@ CXX = os.environ.get("CXX", "g++")
 #   style "share5": argv e1 e2 tX tY itr   (SmithWaterman/EditDistance Share)
 #   style "cache2": argv e1 e2, itr = 100  (Cache drivers, LCS Share)
 #   style "shfl":   argv e1 e2, prints total seconds of 100 calls
+#   style "naive", "tile": argv e1 e2, one call, prints its seconds
 DRIVERS = [
     ("LCS", "GPU_Hyperlane_Share", "LCS", "lcs2", "share"),
     ("LCS", "GPU_Hyperlane_Cache", "LCS", "lcs2", "cache"),
@@ -51,6 +52,15 @@ DRIVERS = [
     ("EditDistance", "GPU_Hyperlane_Cache", "ED", "cache2", "cache"),
     ("LCS-SHFL", "GPU_Hyperlane_Share", "LCS", "shfl", "share"),
     ("LCS-SHFL", "GPU_Hyperlane_Cache", "LCS", "shfl", "cache"),
+    # Naive baselines: GPU (one launch per cell anti-diagonal) and
+    # GPU_TILE_SHARE (one launch per tile anti-diagonal; the reference's
+    # SmithWaterman and EditDistance copies of it compute LCS).
+    ("LCS", "GPU", "LCS", "naive", "naive"),
+    ("SmithWaterman", "GPU", "SW", "naive", "naive"),
+    ("EditDistance", "GPU", "ED", "naive", "naive"),
+    ("LCS", "GPU_TILE_SHARE", "LCS", "tile", "tile"),
+    ("SmithWaterman", "GPU_TILE_SHARE", "LCS", "tile", "tile"),
+    ("EditDistance", "GPU_TILE_SHARE", "LCS", "tile", "tile"),
 ]
 
 
@@ -69,7 +79,7 @@ def build_lib() -> Path:
     LIB.mkdir(exist_ok=True)
     OBJ.mkdir(exist_ok=True)
     srcs = [CSRC / "wf_engine.hip", CSRC / "wf_kernels.hip", CSRC / "wf_api.cpp", CSRC / "wf_server.cpp",
-            CSRC / "wf_tune.cpp"]
+            CSRC / "wf_tune.cpp", CSRC / "wf_naive.hip"]
     srcs = [s for s in srcs if s.exists()]
     objs = [OBJ / (s.stem + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
@@ -87,7 +97,7 @@ def build_compat() -> None:
     src = CSRC / "wf_compat.cpp"
     if not src.exists():
         return
-    for policy in ("share", "cache"):
+    for policy in ("share", "cache", "naive", "tile"):
         out = LIB / f"libwfdp_compat_{policy}.so"
         _run([CXX, "-std=c++17", "-O2", "-fPIC", "-shared", f"-DWF_COMPAT_POLICY_{policy.upper()}",
               f"-I{INCLUDE}", str(src), "-o", str(out), f"-L{LIB}", "-lwfdp",

@@ -660,6 +660,59 @@ int wf_align_table(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const
     return wf_download_results(ctx, &ph, ar, ms, out, 1);
 }
 
+int wf_align_naive(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
+                   const wf_tile *tile, int method, int32_t *table, wf_result *out)
+{
+    if (!ctx || !out || n1 < 0 || n2 < 0 || (n1 > 0 && !a) || (n2 > 0 && !b)) return WF_ERR_INVALID;
+    if (problem != WF_LCS && problem != WF_ED && problem != WF_SW) return WF_ERR_INVALID;
+    if (method != WF_NAIVE_DIAGONAL && method != WF_NAIVE_TILE) return WF_ERR_INVALID;
+    if (n1 >= INT32_MAX || n2 >= INT32_MAX) return WF_ERR_INVALID;
+    const bool textbook = tile && (tile->flags & WF_TILE_ED_TEXTBOOK);
+    *out = wf_result{};
+    out->cells = (uint64_t)n1 * (uint64_t)n2;
+    if (n1 == 0 || n2 == 0) {
+        // One side empty: the table is its boundary.
+        out->last = (problem == WF_ED) ? (int32_t)(n2 > 0 ? n2 : (textbook ? n1 : std::max<int64_t>(0, n1 - 1))) : 0;
+        return WF_OK;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return WF_ERR_DEVICE;
+    const int64_t cells = (n1 + 1) * (n2 + 1);
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return WF_ERR_DEVICE;
+    const size_t need = (size_t)cells * sizeof(int32_t) + (size_t)(n1 + n2 + 64) * sizeof(int32_t);
+    if (need > fr) return WF_ERR_NOMEM;
+    DevBuf dt, dab;
+    int st;
+    if ((st = dt.ensure((size_t)cells * sizeof(int32_t))) != WF_OK) return st;
+    if ((st = dab.ensure((size_t)(n1 + n2 + 64) * sizeof(int32_t))) != WF_OK) return st;
+    int32_t *da = (int32_t *)dab.ptr, *db = da + n1, *smax = db + n2;
+    if (hipMemcpyAsync(da, a, (size_t)n1 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(db, b, (size_t)n2 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(smax, 0, sizeof(int32_t), ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (wf_launch_naive(method, problem, textbook ? 1 : 0, (int *)dt.ptr, da, db, (int)n1, (int)n2,
+                        problem == WF_SW ? smax : nullptr, ctx->stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    const int32_t *dlast = (const int32_t *)dt.ptr + cells - 1;
+    int32_t last = 0, mx = 0;
+    if (hipMemcpy(&last, dlast, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&mx, smax, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (table && hipMemcpy2D(table, (size_t)n1 * sizeof(int32_t), (const int32_t *)dt.ptr + n1 + 2,
+                             (size_t)(n1 + 1) * sizeof(int32_t), (size_t)n1 * sizeof(int32_t), (size_t)n2,
+                             hipMemcpyDeviceToHost) != hipSuccess)
+        return WF_ERR_DEVICE;
+    out->last = last;
+    out->max_score = problem == WF_SW ? mx : 0;
+    out->kernel_ms = ms;
+    return WF_OK;
+}
+
 int wf_plan_create(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile, wf_plan **out)
 {
     if (!ctx || !out || njobs <= 0 || !jobs) return WF_ERR_INVALID;

@@ -1,17 +1,12 @@
 // wf_compat.cpp -- the reference's per-problem C++ entry points over the C
-// ABI.  Built twice: -DWF_COMPAT_POLICY_SHARE and -DWF_COMPAT_POLICY_CACHE.
+// ABI.  Built four times: -DWF_COMPAT_POLICY_SHARE and _CACHE (the hyperlane
+// drivers' functions), _NAIVE and _TILE (the naive baselines' functions).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
 
 #include "wfdp.h"
 #include "wfdp_compat.h"
-
-#if defined(WF_COMPAT_POLICY_CACHE)
-static const int kDefaultTX = 256, kDefaultTY = 1024, kPolicyFlags = WF_TILE_CACHE;
-#else
-static const int kDefaultTX = 32, kDefaultTY = 256, kPolicyFlags = 0;
-#endif
 
 static wf_ctx *g_ctx = nullptr;
 
@@ -20,6 +15,73 @@ static void fail(int st)
     std::printf("GPU error in %s: %s\n", __FILE__, wf_strerror(st));
     std::exit(EXIT_FAILURE);
 }
+
+static wf_ctx *ctx()
+{
+    if (!g_ctx) {
+        const int st = wf_ctx_create(0, &g_ctx);
+        if (st != WF_OK) fail(st);
+    }
+    return g_ctx;
+}
+
+#if defined(WF_COMPAT_POLICY_NAIVE) || defined(WF_COMPAT_POLICY_TILE)
+#if defined(WF_COMPAT_POLICY_NAIVE)
+static const int kMethod = WF_NAIVE_DIAGONAL;
+#else
+static const int kMethod = WF_NAIVE_TILE;
+#endif
+
+// The memory line the reference's naive functions print before they
+// allocate the table (LCS/GPU.cu:51-56): free and total device memory, the
+// padded table's dimensions and its size in bytes.
+static int run_naive(int problem, int n1, int n2, const int *arr1, const int *arr2, int flags)
+{
+    wf_device_info d{};
+    int st = wf_device_query(ctx(), &d);
+    if (st != WF_OK) fail(st);
+    const long long colsize = 1LL + n1, rowsize = 1LL + n2;
+    std::printf("current GPU memory info FREE: %lld Bytes, Total: %lld Bytes.colsize: %lld, rowsize: %lld, "
+                "allocates: %lld Bytes.\n",
+                (long long)d.free_mem, (long long)d.total_mem, colsize, rowsize,
+                colsize * rowsize * (long long)sizeof(int));
+    std::fflush(stdout);
+    wf_tile t{0, 0, 0, flags};
+    wf_result r{};
+    st = wf_align_naive(ctx(), problem, reinterpret_cast<const int32_t *>(arr1), n1,
+                        reinterpret_cast<const int32_t *>(arr2), n2, &t, kMethod, nullptr, &r);
+    if (st != WF_OK) fail(st);
+    return r.last;
+}
+
+// LCS/GPU.h:5 and LCS/GPU_TILE_SHARE.h:5 (whose SmithWaterman and
+// EditDistance copies declare, and compute, LCS as well).
+int LCS(int n1, int n2, int *arr1, int *arr2)
+{
+    return run_naive(WF_LCS, n1, n2, arr1, arr2, 0);
+}
+
+#if defined(WF_COMPAT_POLICY_NAIVE)
+// SmithWaterman/GPU.h:5
+int SW(int n1, int n2, int *arr1, int *arr2)
+{
+    return run_naive(WF_SW, n1, n2, arr1, arr2, 0);
+}
+
+// EditDistance/GPU.h:5; row 0 is the textbook x (EditDistance/GPU.cu:12-18).
+int ED(int n1, int n2, int *arr1, int *arr2)
+{
+    return run_naive(WF_ED, n1, n2, arr1, arr2, WF_TILE_ED_TEXTBOOK);
+}
+#endif
+
+#else  // hyperlane drivers
+
+#if defined(WF_COMPAT_POLICY_CACHE)
+static const int kDefaultTX = 256, kDefaultTY = 1024, kPolicyFlags = WF_TILE_CACHE;
+#else
+static const int kDefaultTX = 32, kDefaultTY = 256, kPolicyFlags = 0;
+#endif
 
 // The reference copies its device table back to `table` only in DEBUG
 // builds (LCS/GPU_Hyperlane_Share.cu:770-772).  Here the environment variable
@@ -40,23 +102,20 @@ static bool debug_table()
 static int run(int problem, int n1, int n2, const int *arr1, const int *arr2, int tX, int tY, int paddX = 0,
                int paddY = 0, int *table = nullptr)
 {
-    if (!g_ctx) {
-        const int st = wf_ctx_create(0, &g_ctx);
-        if (st != WF_OK) fail(st);
-    }
+    wf_ctx *c = ctx();
     wf_tile t{tX, tY, 0, kPolicyFlags};
     wf_result r{};
     const int32_t *a = reinterpret_cast<const int32_t *>(arr1), *b = reinterpret_cast<const int32_t *>(arr2);
     if (table && debug_table() && n1 == n2 && n1 > 0 && (int64_t)n1 * n2 <= WF_TABLE_MAX_CELLS) {
         std::vector<int32_t> h((size_t)n1 * n2);
-        const int st = wf_align_table(g_ctx, problem, a, n1, b, n2, &t, h.data(), &r);
+        const int st = wf_align_table(c, problem, a, n1, b, n2, &t, h.data(), &r);
         if (st != WF_OK) fail(st);
         for (int i = 0; i < n2; i++)
             for (int j = 0; j < n1; j++)
                 table[(int64_t)(i + paddY) * (n2 + paddX) + (j + paddX)] = h[(size_t)i * n1 + j];
         return r.last;
     }
-    const int st = wf_align(g_ctx, problem, a, n1, b, n2, &t, &r);
+    const int st = wf_align(c, problem, a, n1, b, n2, &t, &r);
     if (st != WF_OK) fail(st);
     return r.last;
 }
@@ -95,3 +154,5 @@ int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table)
 {
     return run(WF_ED, n1, n2, arr1, arr2, kDefaultTX, kDefaultTY, paddX, paddY, table);
 }
+
+#endif

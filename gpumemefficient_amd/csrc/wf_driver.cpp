@@ -12,6 +12,12 @@
 //                             EditDistance/GPU_Hyperlane_Cache.cpp: "e1 e2", itr = 100
 //   -DWF_DRIVER_STYLE_SHFL    LCS-SHFL/GPU_Hyperlane_{Share,Cache}.cpp: "e1 e2",
 //                             prints the total seconds of 100 calls
+//   -DWF_DRIVER_STYLE_NAIVE   {LCS,SmithWaterman,EditDistance}/GPU.cpp:47-104:
+//                             "e1 e2", one call, prints its seconds ("lcs
+//                             length" for LCS, "last length" otherwise)
+//   -DWF_DRIVER_STYLE_TILE    */GPU_TILE_SHARE.cpp:47-104: as NAIVE, always LCS
+//                             and "lcs length" (the reference's three copies
+//                             of this driver are identical)
 // The input is ./Data/str1_2_<e1>_str2_2_<e2>.txt; e1 and e2 only name the
 // file, the sizes come from its header (LCS/GPU_Hyperlane_Share.cpp:14-34,
 // 65-79).  The timed region is the loop of host calls, as in the reference.
@@ -113,6 +119,25 @@ int main(int argc, char **argv)
 
     int score = 0;
     timeval tbegin, tend;
+#if defined(WF_DRIVER_STYLE_NAIVE) || defined(WF_DRIVER_STYLE_TILE)
+    gettimeofday(&tbegin, NULL);
+#if defined(WF_DRIVER_STYLE_TILE) || defined(WF_DRIVER_PROBLEM_LCS)
+    score = LCS(n1, n2, arr1.data(), arr2.data());
+#elif defined(WF_DRIVER_PROBLEM_SW)
+    score = SW(n1, n2, arr1.data(), arr2.data());
+#else
+    score = ED(n1, n2, arr1.data(), arr2.data());
+#endif
+    gettimeofday(&tend, NULL);
+    (void)paddX;
+    (void)paddY;
+#if defined(WF_DRIVER_STYLE_TILE) || defined(WF_DRIVER_PROBLEM_LCS)
+    cout << "lcs length: " << score << endl;
+#else
+    cout << "last length: " << score << endl;
+#endif
+    cout << "execution time: " << elapsed_ms(tbegin, tend) / 1000.0 << " second." << endl;
+#else
     gettimeofday(&tbegin, NULL);
     for (int i = 0; i < itr; i++) {
 #if defined(WF_DRIVER_PROBLEM_LCS)
@@ -170,5 +195,6 @@ int main(int argc, char **argv)
             output << '\n';
         }
     }
+#endif
     return 0;
 }

@@ -27,6 +27,10 @@ extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *c
                                       uint32_t *epoch, hipStream_t stream);
 // Lock-step <2,4,32> sweep that also writes every cell of pair 0 to ka->table.
 extern "C" hipError_t wf_launch_table(int grid, const WfKernelArgs *ka, hipStream_t stream);
+// Naive baselines (wf_naive.hip): method 0 one launch per cell
+// anti-diagonal, 1 one launch per tile anti-diagonal; the full table t.
+extern "C" hipError_t wf_launch_naive(int method, int prob, int textbook, int *t, const int32_t *a,
+                                      const int32_t *b, int n1, int n2, int *smax, hipStream_t stream);
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
 

@@ -135,6 +135,23 @@ int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32
 int wf_align_table(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
                    int64_t n2, const wf_tile *tile, int32_t *table, wf_result *out);
 
+/* The reference's naive GPU baselines, for "vs naive" comparisons: the whole
+ * (n2+1) x (n1+1) int32 table lives in HBM and every cell is written there.
+ *   WF_NAIVE_DIAGONAL: one launch per cell anti-diagonal, one thread per cell
+ *     (LCS/GPU.cu:66-102, SmithWaterman/GPU.cu, EditDistance/GPU.cu).
+ *   WF_NAIVE_TILE: one launch per anti-diagonal of 64 x 128 tiles, each tile
+ *     swept in LDS (LCS/GPU_TILE_SHARE.cu:140-182; the reference launches
+ *     each tile separately over 32 streams).
+ * Symbols are compared as int32 (no alphabet limit).  tile: only its
+ * WF_TILE_ED_TEXTBOOK flag is read (the reference's naive ED uses the
+ * textbook row 0, EditDistance/GPU.cu:12-18).  table, if not NULL, receives
+ * table[(y-1)*n1 + (x-1)] = H(y, x) as wf_align_table does.  out->kernel_ms
+ * spans all launches.  WF_ERR_NOMEM when the table does not fit in free HBM. */
+#define WF_NAIVE_DIAGONAL 0
+#define WF_NAIVE_TILE 1
+int wf_align_naive(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
+                   int64_t n2, const wf_tile *tile, int method, int32_t *table, wf_result *out);
+
 int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                    wf_result *out);
 

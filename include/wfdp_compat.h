@@ -35,6 +35,16 @@ int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, i
 /* EditDistance/GPU_Hyperlane_Cache.h:5 */
 int ED(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
 
+/* The naive baselines' functions (libwfdp_compat_naive.so: one launch per
+ * cell anti-diagonal; libwfdp_compat_tile.so: one launch per tile
+ * anti-diagonal, LCS only).  They print the reference's memory line first
+ * (LCS/GPU.cu:51-56).  LCS(n1, n2, arr1, arr2) above is LCS/GPU.h:5 and
+ * LCS/GPU_TILE_SHARE.h:5 in these libraries. */
+/* SmithWaterman/GPU.h:5 */
+int SW(int n1, int n2, int *arr1, int *arr2);
+/* EditDistance/GPU.h:5 (textbook row 0, EditDistance/GPU.cu:12-18) */
+int ED(int n1, int n2, int *arr1, int *arr2);
+
 /* Extension used by the drop-in LCS driver's optional 5-argument form. */
 int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
 

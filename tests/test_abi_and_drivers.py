@@ -30,7 +30,10 @@ DRIVERS = [
     ("SmithWaterman", "GPU_Hyperlane_Share"), ("SmithWaterman", "GPU_Hyperlane_Cache"),
     ("EditDistance", "GPU_Hyperlane_Share"), ("EditDistance", "GPU_Hyperlane_Cache"),
     ("LCS-SHFL", "GPU_Hyperlane_Share"), ("LCS-SHFL", "GPU_Hyperlane_Cache"),
+    ("LCS", "GPU"), ("SmithWaterman", "GPU"), ("EditDistance", "GPU"),
+    ("LCS", "GPU_TILE_SHARE"), ("SmithWaterman", "GPU_TILE_SHARE"), ("EditDistance", "GPU_TILE_SHARE"),
 ]
+HYPERLANE = [d for d in DRIVERS if d[1].startswith("GPU_Hyperlane")]
 
 
 def _header_functions(path: Path) -> set[str]:
@@ -58,6 +61,13 @@ def test_compat_libraries_export_reference_signatures():
         out = subprocess.run(["nm", "-D", "--defined-only", str(p)], capture_output=True, text=True,
                              check=True).stdout
         for sym in want:
+            assert sym in out, f"{p.name} lacks {sym}"
+    # The naive baselines' four-argument functions (GPU.h, GPU_TILE_SHARE.h).
+    for policy, syms in (("naive", ["_Z3LCSiiPiS_", "_Z2SWiiPiS_", "_Z2EDiiPiS_"]), ("tile", ["_Z3LCSiiPiS_"])):
+        p = LIBDIR / f"libwfdp_compat_{policy}.so"
+        out = subprocess.run(["nm", "-D", "--defined-only", str(p)], capture_output=True, text=True,
+                             check=True).stdout
+        for sym in syms:
             assert sym in out, f"{p.name} lacks {sym}"
 
 
@@ -119,7 +129,7 @@ def _stage(tmp_path: Path, e: int) -> None:
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pdir,name", DRIVERS)
+@pytest.mark.parametrize("pdir,name", HYPERLANE)
 @pytest.mark.parametrize("e", [8, 10, 12])
 def test_driver_stdout_and_score(tmp_path, pdir, name, e):
     _stage(tmp_path, e)
