@@ -22,7 +22,9 @@ One JSON line on stdout:
                launch per pair; the reference's naive baselines (one launch
                per cell or tile anti-diagonal) on LCS 8192^2; the served
                Poisson stream (p50/p99 latency and GCUPS per rate, continuous
-               batching against one launch per request)
+               batching against one launch per request); the SAT and 2D-SOR
+               table sweeps at 2^15 x 2^15 (the reference's nvprof runs of
+               those programs) against the HBM bandwidth they are bound by
 """
 from __future__ import annotations
 
@@ -88,6 +90,40 @@ def make_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def stencil_measure(ctx, exp: int = 15, steps: int = 3, warmup: int = 1, seed: int = 6) -> dict:
+    """SAT and 2D-SOR sweeps of a (2^exp + 2) x (2^exp + 3) table of uniform
+    [0, 10000) values (the reference generator's range), resident in HBM.
+    Algorithmic HBM traffic: every interior cell read once and written once,
+    8 bytes per cell."""
+    import numpy as np
+    from gpumemefficient_amd._hip import DeviceBuffer
+    n = 1 << exp
+    ld = (n + 3 + 3) & ~3
+    host = np.random.default_rng(seed).integers(0, 10000, size=(n + 2, ld), dtype=np.int32)
+    dev = DeviceBuffer.from_array(host)
+    out = {"workload": f"table_{n}x{n}_uniform10000_seed{seed}", "n1": n, "n2": n, "bytes_per_cell": 8}
+    for op in ("SAT", "SOR"):
+        dev.upload(host)
+        for _ in range(warmup):
+            ctx.stencil_device(op, dev.ptr, n, n, ld)
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            kms.append(ctx.stencil_device(op, dev.ptr, n, n, ld).kernel_ms)
+        dt = time.perf_counter() - t0
+        k = sum(kms) / len(kms)
+        cells = n * n
+        gbs = 8 * cells / (k * 1e-3) / 1e9
+        out[op.lower()] = {"kernel_ms": round(k, 3), "gcups": round(cells / k / 1e6, 1),
+                           "wall_gcups": round(cells * steps / dt / 1e9, 1),
+                           "hbm_gbs": round(gbs, 1), "hbm_peak_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dev.free()
+    return out
+
+
 def time_plan(plan, steps: int, warmup: int, per_pair: bool = False):
     for _ in range(warmup):
         plan.run(per_pair=per_pair)
@@ -130,7 +166,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ed", choices=["ed", "sw", "lcs", "batch"])
+    ap.add_argument("--workload", default="ed", choices=["ed", "sw", "lcs", "batch", "stencil"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the SW single-pair, batch and serving measurements")
     ap.add_argument("--serve-requests", type=int, default=600, help="requests per rate in the serving measurement")
@@ -145,6 +181,11 @@ def main() -> int:
     ctx = Context(0, timeout_s=30.0)
     info = ctx.info()
     peak = peak_int32_tops(info)
+    if args.workload == "stencil":
+        # The SAT and 2D-SOR sweeps alone (for profiling); one JSON line.
+        print(json.dumps({"stencil_32768": stencil_measure(ctx, steps=args.steps, warmup=args.warmup)}))
+        ctx.close()
+        return 0
 
     jobs, prob, config = make_workload(args.workload)
     cells = sum(int(j.a.size) * int(j.b.size) for j in jobs)
@@ -244,6 +285,7 @@ def main() -> int:
         sat_p = serve["per_request"][-1]["gcups"]
         serve["saturated_speedup"] = round(sat_c / sat_p, 2) if sat_p else None
         extra["serving"] = serve
+        extra["stencil_32768"] = stencil_measure(ctx)
         line["extra"] = extra
 
     if not args.no_cpu:

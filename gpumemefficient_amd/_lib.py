@@ -33,13 +33,17 @@ WF_TABLE_MAX_CELLS = 1 << 28
 WF_NAIVE_DIAGONAL = 0
 WF_NAIVE_TILE = 1
 
+WF_SAT, WF_SOR = 0, 1
+STENCILS = {"SAT": WF_SAT, "SOR": WF_SOR}
+
 WF_RUN_PERSISTENT = 0
 WF_RUN_PER_PAIR = 1
 
 # Every symbol include/wfdp.h declares (checked by the CPU test suite).
 EXPORTS = [
     "wf_ctx_create", "wf_ctx_destroy", "wf_device_query", "wf_strerror", "wf_abi_version",
-    "wf_ctx_set_timeout", "wf_align", "wf_align_table", "wf_align_naive", "wf_align_batch", "wf_plan_create", "wf_plan_run",
+    "wf_ctx_set_timeout", "wf_align", "wf_align_table", "wf_align_naive", "wf_stencil", "wf_stencil_device",
+    "wf_align_batch", "wf_plan_create", "wf_plan_run",
     "wf_plan_results", "wf_plan_kernel_ms", "wf_plan_describe", "wf_plan_destroy",
     "wf_variant_count", "wf_variant_info", "wf_plan_capture", "wf_autotune", "wf_tune_load", "wf_tune_save",
     "wf_tune_clear", "wf_server_create", "wf_server_submit", "wf_server_poll", "wf_server_wait_idle",
@@ -108,6 +112,12 @@ def load() -> C.CDLL:
     lib.wf_align_naive.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32), C.c_int64,
                                    C.POINTER(wf_tile), C.c_int, C.POINTER(C.c_int32), C.POINTER(wf_result)]
     lib.wf_align_naive.restype = C.c_int
+    lib.wf_stencil.argtypes = [P, C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_int32), C.c_int64,
+                               C.POINTER(wf_tile), C.POINTER(wf_result)]
+    lib.wf_stencil.restype = C.c_int
+    lib.wf_stencil_device.argtypes = [P, C.c_int, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                                      C.POINTER(wf_tile), C.POINTER(wf_result)]
+    lib.wf_stencil_device.restype = C.c_int
     lib.wf_align_batch.argtypes = [P, C.POINTER(wf_job), C.c_int64, C.POINTER(wf_tile), C.POINTER(wf_result)]
     lib.wf_align_batch.restype = C.c_int
     lib.wf_plan_create.argtypes = [P, C.POINTER(wf_job), C.c_int64, C.POINTER(wf_tile), C.POINTER(P)]

@@ -34,6 +34,15 @@ def _problem_id(problem) -> int:
     return p
 
 
+def _stencil_id(op) -> int:
+    if isinstance(op, str):
+        try:
+            return _lib.STENCILS[op.upper()]
+        except KeyError:
+            raise ValueError(f"unknown stencil {op!r}; expected one of {sorted(_lib.STENCILS)}") from None
+    return int(op)
+
+
 def _as_i32(x) -> np.ndarray:
     arr = np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1))
     return arr
@@ -211,6 +220,30 @@ class Context:
               "wf_align_naive")
         return Result._from(out), table
 
+    def stencil(self, op, table, strip_rows: int = 0) -> tuple[Result, np.ndarray]:
+        """One in-place sweep of the reference's SAT or 2D-SOR stencil ("SAT"
+        / "SOR") over an (n2 + 2, n1 + 3) int32 table (boundary row 0, row
+        n2 + 1, columns 0, n1 + 1, n1 + 2).  Returns the result and the swept
+        copy; `table` is not modified."""
+        t = np.array(table, dtype=np.int32, copy=True, order="C")
+        if t.ndim != 2 or t.shape[0] < 2 or t.shape[1] < 3:
+            raise ValueError("table must be (n2 + 2, n1 + 3) with n1, n2 >= 0")
+        n2, n1 = t.shape[0] - 2, t.shape[1] - 3
+        tl = _lib.wf_tile(0, int(strip_rows), 0, 0)
+        out = _lib.wf_result()
+        check(self._lib.wf_stencil(self._h, _stencil_id(op), n1, n2, _ptr(t), t.shape[1], C.byref(tl),
+                                   C.byref(out)), "wf_stencil")
+        return Result._from(out), t
+
+    def stencil_device(self, op, dev_ptr: int, n1: int, n2: int, ld: int, strip_rows: int = 0) -> Result:
+        """The same sweep on a table already in HBM (a device pointer, e.g. a
+        torch tensor's data_ptr(); pitch ld % 4 == 0, 16-byte aligned)."""
+        tl = _lib.wf_tile(0, int(strip_rows), 0, 0)
+        out = _lib.wf_result()
+        check(self._lib.wf_stencil_device(self._h, _stencil_id(op), int(n1), int(n2), C.c_void_p(int(dev_ptr)),
+                                          int(ld), C.byref(tl), C.byref(out)), "wf_stencil_device")
+        return Result._from(out)
+
     def align_batch(self, jobs: Iterable[Job], tile: Tile | None = None) -> list[Result]:
         ja = _JobArray(jobs)
         out = (_lib.wf_result * max(1, ja.n))()
@@ -327,5 +360,20 @@ def ED(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
     return default_context().align("ED", a, b, t).last
 
 
-__all__ = ["Context", "Plan", "Job", "Tile", "Result", "WfError", "LCS", "SW", "ED", "default_context", "variants",
-           "tile_for"]
+def SOR(n1: int, n2: int, table, tX: int | None = None, tY: int | None = None, kind: str = "SOR") -> np.ndarray:
+    """Reference signature SOR(n1, n2, table[, tX, tY]) of the hyperlane SAT
+    and 2D-SOR programs (SAT/GPU_Hyperlane_Share.h:5,
+    2D-SOR/nested_loop/GPU_Hyperlane_Share.h:5), whose host function is named
+    SOR in both; kind="SAT" selects the summed-area table.  `table` holds at
+    least (n2 + 2) rows of n1 + 3 values (the reference reads n2 + 3 rows,
+    SAT/GPU_Hyperlane_Share.cpp:23-29, and uploads n2 + 2).  The reference
+    leaves `table` unchanged (its copy back is commented out,
+    SAT/GPU_Hyperlane_Share.cu:317); this returns the swept (n2 + 2, n1 + 3)
+    table instead.  tX and tY only shape the reference's tiles."""
+    del tX, tY
+    t = np.asarray(table, dtype=np.int32).reshape(-1)[: (n2 + 2) * (n1 + 3)].reshape(n2 + 2, n1 + 3)
+    return default_context().stencil(kind, t)[1]
+
+
+__all__ = ["Context", "Plan", "Job", "Tile", "Result", "WfError", "LCS", "SW", "ED", "SOR", "default_context",
+           "variants", "tile_for"]

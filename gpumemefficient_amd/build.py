@@ -7,7 +7,10 @@ Outputs (git-ignored, shipped with the source tree):
     gpumemefficient_amd/lib/libwfdp.so          C ABI of include/wfdp.h (HIP, gfx950)
     gpumemefficient_amd/lib/libwfdp_compat_share.so, libwfdp_compat_cache.so
                                                 the reference's C++ signatures
-    gpumemefficient_amd/bin/<Problem>/<driver>  drop-in drivers
+    gpumemefficient_amd/lib/libwfdp_compat_sat.so, libwfdp_compat_sor.so
+                                                SOR() of the SAT and 2D-SOR programs
+    gpumemefficient_amd/bin/<Problem>/<driver>  drop-in drivers (LCS, SmithWaterman,
+                                                EditDistance, LCS-SHFL, SAT, 2D-SOR)
     oracle/liboracle.so, oracle/_ref/*          checker side (oracle/Makefile)
 """
 from __future__ import annotations
@@ -79,7 +82,7 @@ def build_lib() -> Path:
     LIB.mkdir(exist_ok=True)
     OBJ.mkdir(exist_ok=True)
     srcs = [CSRC / "wf_engine.hip", CSRC / "wf_kernels.hip", CSRC / "wf_api.cpp", CSRC / "wf_server.cpp",
-            CSRC / "wf_tune.cpp", CSRC / "wf_naive.hip"]
+            CSRC / "wf_tune.cpp", CSRC / "wf_naive.hip", CSRC / "wf_stencil.hip", CSRC / "wf_stencil_api.cpp"]
     srcs = [s for s in srcs if s.exists()]
     objs = [OBJ / (s.stem + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
@@ -97,7 +100,7 @@ def build_compat() -> None:
     src = CSRC / "wf_compat.cpp"
     if not src.exists():
         return
-    for policy in ("share", "cache", "naive", "tile"):
+    for policy in ("share", "cache", "naive", "tile", "sat", "sor"):
         out = LIB / f"libwfdp_compat_{policy}.so"
         _run([CXX, "-std=c++17", "-O2", "-fPIC", "-shared", f"-DWF_COMPAT_POLICY_{policy.upper()}",
               f"-I{INCLUDE}", str(src), "-o", str(out), f"-L{LIB}", "-lwfdp",
@@ -123,6 +126,28 @@ def build_drivers() -> None:
             _run([CXX, "-std=c++17", "-O2", f"-DWF_GEN_MIN_EXP={minexp}", str(gen), "-o", str(d / "generator")])
 
 
+# Stencil drivers: (program directory, binary name, compat library, style)
+STENCIL_DRIVERS = [
+    ("SAT", "GPU_Hyperlane_Share", "sat", "share5"),
+    ("SAT", "GPU_Hyperlane_Cache", "sat", "cache2"),
+    ("2D-SOR", "GPU_Hyperlane_Share", "sor", "share5"),
+    ("2D-SOR", "GPU_Hyperlane_Cache", "sor", "cache2"),
+]
+
+
+def build_stencil_drivers(min_exp: int = 13) -> None:
+    src = CSRC / "wf_stencil_driver.cpp"
+    for pdir, name, lib, style in STENCIL_DRIVERS:
+        d = BIN / pdir
+        d.mkdir(parents=True, exist_ok=True)
+        flags = ["-DWF_SDRIVER_SHARE5"] if style == "share5" else []
+        _run([CXX, "-std=c++17", "-O2", *flags, f'-DWF_SDRIVER_NAME="{pdir}"', f"-I{INCLUDE}", str(src),
+              "-o", str(d / name), f"-L{LIB}", f"-lwfdp_compat_{lib}", "-lwfdp", "-Wl,-rpath,$ORIGIN/../../lib"])
+    gen = CSRC / "wf_stencil_generator.cpp"
+    for pdir in ("SAT", "2D-SOR"):
+        _run([CXX, "-std=c++17", "-O2", f"-DWF_GEN_MIN_EXP={min_exp}", str(gen), "-o", str(BIN / pdir / "generator")])
+
+
 def build_oracle() -> None:
     if shutil.which("make") is None:
         raise RuntimeError("make not found")
@@ -135,6 +160,7 @@ def build_all(lib_only: bool = False) -> None:
         return
     build_compat()
     build_drivers()
+    build_stencil_drivers()
     build_oracle()
 
 

@@ -1,6 +1,7 @@
 // wf_compat.cpp -- the reference's per-problem C++ entry points over the C
-// ABI.  Built four times: -DWF_COMPAT_POLICY_SHARE and _CACHE (the hyperlane
-// drivers' functions), _NAIVE and _TILE (the naive baselines' functions).
+// ABI.  Built six times: -DWF_COMPAT_POLICY_SHARE and _CACHE (the hyperlane
+// drivers' functions), _NAIVE and _TILE (the naive baselines' functions),
+// _SAT and _SOR (the SAT and 2D-SOR programs' SOR()).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -25,7 +26,53 @@ static wf_ctx *ctx()
     return g_ctx;
 }
 
-#if defined(WF_COMPAT_POLICY_NAIVE) || defined(WF_COMPAT_POLICY_TILE)
+// The reference copies its device table back to the host only in DEBUG
+// builds (LCS/GPU_Hyperlane_Share.cu:770-772; the SAT and 2D-SOR hosts not
+// even there, SAT/GPU_Hyperlane_Share.cu:317).  The environment variable
+// WFDP_DEBUG_TABLE=1 plays that role here.
+static bool debug_table()
+{
+    static const int on = [] {
+        const char *e = std::getenv("WFDP_DEBUG_TABLE");
+        return (e && *e && *e != '0') ? 1 : 0;
+    }();
+    return on != 0;
+}
+
+#if defined(WF_COMPAT_POLICY_SAT) || defined(WF_COMPAT_POLICY_SOR)
+#if defined(WF_COMPAT_POLICY_SAT)
+static const int kStencil = WF_SAT;
+#else
+static const int kStencil = WF_SOR;
+#endif
+
+// One sweep of the (n2 + 2) x (n1 + 3) table the driver read (it holds n2 + 3
+// rows; the reference uploads n2 + 2, SAT/GPU_Hyperlane_Share.cu:261-278).
+// The reference discards the result; with WFDP_DEBUG_TABLE=1 it is written
+// back into `table`.  tX and tY only shape the reference's tiles.
+static void run_stencil(int n1, int n2, int *table)
+{
+    wf_tile t{0, 0, 0, debug_table() ? 0 : WF_TILE_NO_COPYBACK};
+    wf_result r{};
+    const int st = wf_stencil(ctx(), kStencil, n1, n2, reinterpret_cast<int32_t *>(table), (int64_t)n1 + 3, &t, &r);
+    if (st != WF_OK) fail(st);
+}
+
+// SAT/GPU_Hyperlane_Share.h:5, 2D-SOR/nested_loop/GPU_Hyperlane_Share.h:5
+void SOR(int n1, int n2, int *table, int tX, int tY)
+{
+    (void)tX;
+    (void)tY;
+    run_stencil(n1, n2, table);
+}
+
+// SAT/GPU_Hyperlane_Cache.h:5, 2D-SOR/nested_loop/GPU_Hyperlane_Cache.h:5
+void SOR(int n1, int n2, int *table)
+{
+    run_stencil(n1, n2, table);
+}
+
+#elif defined(WF_COMPAT_POLICY_NAIVE) || defined(WF_COMPAT_POLICY_TILE)
 #if defined(WF_COMPAT_POLICY_NAIVE)
 static const int kMethod = WF_NAIVE_DIAGONAL;
 #else
@@ -83,21 +130,11 @@ static const int kDefaultTX = 256, kDefaultTY = 1024, kPolicyFlags = WF_TILE_CAC
 static const int kDefaultTX = 32, kDefaultTY = 256, kPolicyFlags = 0;
 #endif
 
-// The reference copies its device table back to `table` only in DEBUG
-// builds (LCS/GPU_Hyperlane_Share.cu:770-772).  Here the environment variable
-// WFDP_DEBUG_TABLE=1 plays that role: with it, a non-null table of a square
-// pair receives the whole DP table in the reference's padded layout,
+// With WFDP_DEBUG_TABLE=1 a non-null table of a square pair receives the
+// whole DP table in the reference's padded layout,
 // table[(i + paddY) * (n2 + paddX) + (j + paddX)] = H(i + 1, j + 1), which its
 // drivers' DEBUG dump prints (LCS/GPU_Hyperlane_Share.cpp:109-131).  Without
 // it the score-only path runs and `table` is left alone.
-static bool debug_table()
-{
-    static const int on = [] {
-        const char *e = std::getenv("WFDP_DEBUG_TABLE");
-        return (e && *e && *e != '0') ? 1 : 0;
-    }();
-    return on != 0;
-}
 
 static int run(int problem, int n1, int n2, const int *arr1, const int *arr2, int tX, int tY, int paddX = 0,
                int paddY = 0, int *table = nullptr)

@@ -63,4 +63,25 @@ struct WfKernelArgs {
     int32_t *table;          // table-dump kernel only: H(y+1, x+1) of pair 0 at [y * n1 + x]
 };
 
+// In-place table stencils swept with the same strip schedule (wf_stencil.hip):
+// the reference's summed-area table and one Gauss-Seidel SOR sweep.
+enum { WFS_SAT = 0, WFS_SOR = 1 };
+
+// Kernel argument block of a stencil sweep.  The table is row-major with
+// pitch `ld` int32 (ld >= n1 + 3, ld % 4 == 0, 16-byte aligned base); rows
+// 0 and n2 + 1 and columns 0, n1 + 1, n1 + 2 are fixed boundary values and
+// the interior 1 <= y <= n2, 1 <= x <= n1 is updated in place.
+struct WfStencilArgs {
+    int32_t *t;
+    int64_t ld;
+    int32_t n1, n2;
+    int32_t nstrips;         // strips of 64 * waves rows
+    int32_t op;              // WFS_SAT / WFS_SOR
+    uint64_t *bnd;           // (nstrips - 1) rows of n1 + 1 words: (epoch << 32) | value
+    int32_t *counter;        // strip claim counter (zeroed by the reset kernel)
+    WfResultDev *res;        // res[0].status: timeout flag
+    const uint32_t *epoch;
+    int64_t timeout_ticks;
+};
+
 #endif

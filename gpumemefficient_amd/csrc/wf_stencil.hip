@@ -1,0 +1,352 @@
+// wf_stencil.hip -- gfx950 wavefront engine for the reference's other users of
+// the hyperlane schedule: in-place sweeps over a whole int32 table whose cell
+// depends on its left and upper neighbours after they were updated.
+//
+//   SAT  summed-area table:
+//          T(y,x) = T(y,x-1) + T(y-1,x) + T(y,x) - T(y-1,x-1)
+//        (SAT/GPU_Hyperlane_Share.cu:169, :196, :233), int32 wrap-around.
+//   SOR  one Gauss-Seidel sweep of the 5-point stencil:
+//          T(y,x) = (T(y,x-1) + T(y-1,x) + T(y,x) + T(y,x+1) + T(y+1,x)) / 5
+//        with left and up already updated and right and down not
+//        (2D-SOR/nested_loop/GPU_Hyperlane_Share.cu:169, :197, :233); C
+//        division truncating toward zero.
+//
+// The table has the reference's layout (SAT/GPU_Hyperlane_Share.cu:255-263):
+// rows 0 .. n2+1 of pitch ld >= n1+3; row 0, row n2+1 and columns 0, n1+1,
+// n1+2 are boundary values that are read, never written; the interior
+// 1 <= y <= n2, 1 <= x <= n1 is updated.  Any order that visits each cell
+// after its left and upper neighbours and before its right and lower ones
+// gives the row-major sequential sweep's result; the reference's hyperlane
+// tiles are one such order and so is this engine's.
+//
+// Geometry.  A strip of 64*W rows is swept left to right by one workgroup of
+// W waves; lane l of a wave owns one row and computes column x = s + 1 - l at
+// step s (the systolic front of the alignment engines).  The updated left
+// value stays in a VGPR, the updated upper value comes from lane l-1 through
+// one DPP wave_shr:1 (lane 0: from an LDS ring that the wave above, or for the
+// strip's first wave the strip above through HBM, fills).  Old values (self,
+// right, and the row below) come from an LDS window of the wave's 64 rows plus
+// the row below, staged from HBM in 32-column blocks with coalesced 16-byte
+// loads one chunk ahead of use; each new value is written back into the
+// window in place, and finished blocks go to HBM with coalesced 16-byte
+// stores.  Every cell is read once from and written once to HBM: the sweep
+// is bound by HBM bandwidth, not by the VALU.
+//
+// Waves of a workgroup advance in lock-step chunks of K = 32 steps between
+// s_barriers, wave w D = 3 chunks behind wave w-1.  A strip hands its bottom
+// row (columns 0..n1) to the next strip through HBM as (launch epoch << 32 |
+// value) words written and polled with relaxed device-scope atomics, as the
+// alignment engines do; strips are claimed in order from an atomic counter,
+// so a strip only waits on one that a running workgroup holds.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wf_cells.h"
+#include "wf_device.h"
+
+namespace wfs {
+
+using wfk::kDppWaveShr1;
+using wfk::now_ticks;
+
+constexpr int kK = 32;                  // steps per chunk
+constexpr int kD = (kK + 62) / kK + 1;  // chunks between consecutive waves
+constexpr int kTW = 128;                // window columns per row (4 blocks of 32)
+constexpr int kTS = kTW + 8;            // row stride in dwords: slot 128 mirrors slot 0
+constexpr int kTR = 65;                 // window rows: the wave's 64 plus the row below
+constexpr int kRS = 256;                // ring slots; column x lives in slot (x + 63) mod kRS
+constexpr int kVec = kTR * 8;           // 16-byte vectors in one block of the window
+
+template <int W>
+struct StShared {
+    // Window rows r = 0..64 of each wave; column x in slot x mod 128.  The
+    // stride of 136 dwords makes the lanes' skewed reads (column s + 1 - l in
+    // row l) fall in distinct banks.
+    __attribute__((aligned(16))) int win[W][kTR * kTS];
+    // ring[0]: the strip's top row (wave 0's lane 0 input); ring[w+1]: lane
+    // 63 of wave w, read by wave w+1 (w = W-1: copied to HBM).
+    __attribute__((aligned(16))) int ring[W + 1][kRS];
+    // Lanes 0..62 store their value here so the ring capture is one unmasked
+    // ds_write per step.
+    int dummy[W][64 + kK];
+    int task;
+    int abort_at;
+};
+
+// One block of the window (columns [32b, 32b+32) of the wave's 65 rows) in
+// flight between its global loads and its LDS stores: vector v = lane + 64k
+// is row v / 8, columns 32b + 4 (v % 8) .. + 3.
+struct Stage {
+    int4 v[(kVec + 63) / 64];
+};
+
+__device__ __forceinline__ void stage_load(Stage &sg, const WfStencilArgs &a, int yw, int b, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < (kVec + 63) / 64; ++k) {
+        const int v = lane + 64 * k;
+        const int rr = v >> 3;
+        const int c = 32 * b + 4 * (v & 7);
+        int4 val = make_int4(0, 0, 0, 0);
+        // The pitch is a multiple of 4, so a vector is inside a row or past it.
+        if (v < kVec && yw + rr <= a.n2 + 1 && c < a.ld)
+            val = *reinterpret_cast<const int4 *>(a.t + (int64_t)(yw + rr) * a.ld + c);
+        sg.v[k] = val;
+    }
+}
+
+__device__ __forceinline__ void stage_store(const Stage &sg, int *win, int b, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < (kVec + 63) / 64; ++k) {
+        const int v = lane + 64 * k;
+        if (v < kVec) {
+            const int rr = v >> 3;
+            const int slot = (32 * b + 4 * (v & 7)) & (kTW - 1);
+            *reinterpret_cast<int4 *>(win + rr * kTS + slot) = sg.v[k];
+            if (slot == 0) *reinterpret_cast<int4 *>(win + rr * kTS + kTW) = sg.v[k];
+        }
+    }
+}
+
+// Store block b of the wave's interior rows (r < nrows) back to the table;
+// vectors that hold no interior column are skipped (boundary columns inside
+// a stored vector are written back unchanged).
+__device__ __forceinline__ void flush_block(const WfStencilArgs &a, const int *win, int yw, int b, int nrows,
+                                            int lane)
+{
+    if (b < 0 || 32 * b > a.n1) return;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v = lane + 64 * k;
+        const int rr = v >> 3;
+        const int c = 32 * b + 4 * (v & 7);
+        const int4 val = *reinterpret_cast<const int4 *>(win + rr * kTS + (c & (kTW - 1)));
+        if (rr < nrows && c + 3 >= 1 && c <= a.n1)
+            *reinterpret_cast<int4 *>(a.t + (int64_t)(yw + rr) * a.ld + c) = val;
+    }
+}
+
+// One chunk of kK steps from s0 for one wave.  `h` is the lane's last updated
+// value (its left neighbour), `dg` the upper neighbour of the previous step
+// (SAT's diagonal).  MASKED: some lane is outside its columns 1..n1 or holds
+// no interior row; those lanes keep their state and write nothing.
+template <int OP, bool MASKED>
+__device__ __forceinline__ void st_chunk(int s0, int lane, int n1, bool rowok, int &h, int &dg, int *wrow,
+                                         const int *ring_prev, int *ring_cur, int *dummy)
+{
+    const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (kRS - 1));
+    int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (kRS - 1)) : dummy + lane;
+    int *cap_last = (lane == 63) ? ring_cur + ((s0 + kK) & (kRS - 1)) : dummy + lane + (kK - 1);
+#pragma unroll
+    for (int q = 0; q < kK; q += 4) {
+        const int4 inj4 = *reinterpret_cast<const int4 *>(inj_base + q);
+        const int injv[4] = {inj4.x, inj4.y, inj4.z, inj4.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = q + u;
+            const int x = s0 + j + 1 - lane;
+            const int p = x & (kTW - 1);
+            const int self = wrow[p];
+            const int up = __builtin_amdgcn_update_dpp(injv[u], h, kDppWaveShr1, 0xf, 0xf, false);
+            int nv;
+            if (OP == WFS_SAT) {
+                nv = (int)((uint32_t)h + (uint32_t)up + (uint32_t)self - (uint32_t)dg);
+            } else {
+                const int right = wrow[p + 1];
+                const int down = wrow[p + kTS];
+                const int sum = (int)((uint32_t)h + (uint32_t)up + (uint32_t)self + (uint32_t)right +
+                                      (uint32_t)down);
+                nv = sum / 5;
+            }
+            bool valid = true;
+            if (MASKED) valid = rowok && (unsigned)(x - 1) < (unsigned)n1;
+            if (valid) wrow[p] = nv;
+            h = valid ? nv : h;
+            dg = up;
+            if (j == kK - 1)
+                *cap_last = h;
+            else
+                cap[j] = h;
+        }
+    }
+}
+
+template <int OP, int W>
+__device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, int lane, int wave, uint32_t epoch)
+{
+    const int n1 = a.n1;
+    const int yw = 1 + g * 64 * W + wave * 64;  // the wave's first row
+    const int rows_left = a.n2 - g * 64 * W;     // interior rows from the strip's first
+    const int wa = min(W, (rows_left + 63) / 64);
+    const bool active = wave < wa;
+    const int nrows = active ? min(64, a.n2 - yw + 1) : 0;  // interior rows of the wave
+    const bool rowok = lane < nrows;
+    const int nsteps = n1 + 63;                 // lane 63 updates column n1 at step n1 + 62
+    const int nch = (nsteps + kK - 1) / kK;
+    const int total = nch + (wa - 1) * kD;
+    const int nfill = (n1 + 1 + kK - 1) / kK;   // ring chunks covering columns 0..n1
+    const bool has_prev = g > 0;
+    const bool has_next = g + 1 < a.nstrips;
+    const uint64_t *bprev = a.bnd + (int64_t)(g - 1) * (n1 + 1);
+    uint64_t *bnext = a.bnd + (int64_t)g * (n1 + 1);
+    const uint64_t tag = (uint64_t)epoch << 32;
+    int *win = sh.win[wave];
+    int *wrow = win + lane * kTS;
+
+    // Top row (columns [32c, 32c+32), wave 0, lanes < 32): row 0 of the table
+    // for the first strip, else the previous strip's bottom row.
+    auto load_top = [&](int c) -> uint64_t {
+        const int x = c * kK + lane;
+        if (lane >= kK || x > n1) return tag;
+        if (!has_prev) return tag | (uint32_t)a.t[x];
+        return __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto fill = [&](int c, uint64_t top) -> bool {
+        const int x = c * kK + lane;
+        const bool mine = lane < kK && x <= n1;
+        if (__builtin_amdgcn_ballot_w64(mine && (top >> 32) != (uint64_t)epoch) != 0) {
+            const int64_t t0 = now_ticks();
+            for (;;) {
+                __builtin_amdgcn_s_sleep(1);
+                if (mine && (top >> 32) != (uint64_t)epoch)
+                    top = __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__builtin_amdgcn_ballot_w64(mine && (top >> 32) != (uint64_t)epoch) == 0) break;
+                if (now_ticks() - t0 > a.timeout_ticks) return false;
+            }
+        }
+        if (mine) sh.ring[0][(x + 63) & (kRS - 1)] = (int)(uint32_t)top;
+        return true;
+    };
+
+    // Prologue: block 0 of the window in LDS, block 1 in flight; the top
+    // ring's first two chunks.
+    Stage sg;
+    if (active) {
+        stage_load(sg, a, yw, 0, lane);
+        stage_store(sg, win, 0, lane);
+        stage_load(sg, a, yw, 1, lane);
+    }
+    if (wave == 0) {
+        bool ok = fill(0, load_top(0));
+        if (ok && nfill > 1) ok = fill(1, load_top(1));
+        if (!ok && lane == 0) sh.abort_at = -1;
+    }
+    __syncthreads();
+
+    int h = active ? wrow[0] : 0;  // column 0: the left boundary
+    int dg = 0;
+    int copied = 0;  // wave W-1: bottom-row columns stored to bnext
+    auto copy_out = [&](int upto, bool from_ring) {
+        while (copied < upto) {
+            const int x = copied + lane;
+            if (x < upto) {
+                const int v = from_ring ? sh.ring[W][(x + 63) & (kRS - 1)] : 0;
+                __hip_atomic_store(bnext + x, tag | (uint32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            copied = min(upto, copied + 64);
+        }
+    };
+
+    bool aborted = sh.abort_at <= -1;
+    for (int j = 0; j < total && !aborted; ++j) {
+        const int cj = j - wave * kD;
+        uint64_t top_pre = tag;
+        if (wave == 0 && cj + 2 < nfill) top_pre = load_top(cj + 2);
+        if (active && cj >= 0 && cj < nch) {
+            const int s0 = cj * kK;
+            // Window: block cj+1 (loaded during the previous chunk) takes the
+            // slot of block cj-3, flushed at the end of the previous chunk;
+            // block cj+2 is loaded for the next chunk.
+            stage_store(sg, win, cj + 1, lane);
+            stage_load(sg, a, yw, cj + 2, lane);
+            if (cj == 0 && lane == 0) dg = sh.ring[wave][63];  // column 0 of the row above
+            const bool steady = nrows == 64 && s0 >= 63 && s0 + kK - 1 <= n1 - 1;
+            if (steady)
+                st_chunk<OP, false>(s0, lane, n1, rowok, h, dg, wrow, sh.ring[wave], sh.ring[wave + 1],
+                                    sh.dummy[wave]);
+            else
+                st_chunk<OP, true>(s0, lane, n1, rowok, h, dg, wrow, sh.ring[wave], sh.ring[wave + 1],
+                                   sh.dummy[wave]);
+            // Steps < s0 + kK are done: lane 63 has finished columns < s0 + kK - 62,
+            // so every lane has finished block cj-2.
+            if (wave == W - 1 && has_next) copy_out(min(n1 + 1, s0 + kK - 62), true);
+            flush_block(a, win, yw, cj - 2, nrows, lane);
+        }
+        if (wave == 0 && cj + 2 < nfill) {
+            if (!fill(cj + 2, top_pre) && lane == 0) sh.abort_at = j;
+        }
+        __syncthreads();
+        aborted = sh.abort_at <= j;
+    }
+
+    if (aborted) {
+        // A dependency wait timed out: mark the launch and release the strip
+        // below with placeholder values so it does not wait out its own timeout.
+        if (threadIdx.x == 0) atomicOr(&a.res->status, 1);
+        if (wave == W - 1 && has_next) copy_out(n1 + 1, false);
+        return;
+    }
+    if (wave == W - 1 && has_next) copy_out(n1 + 1, true);
+    // Blocks not yet flushed: nch-2 .. the one holding column n1 (all of
+    // them are still in the window).
+    if (active)
+        for (int b = max(0, nch - 2); 32 * b <= n1; ++b) flush_block(a, win, yw, b, nrows, lane);
+}
+
+template <int OP, int W>
+__global__ __launch_bounds__(64 * W) void wf_stencil_kernel(WfStencilArgs a)
+{
+    __shared__ StShared<W> sh;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t epoch = __builtin_amdgcn_readfirstlane(*a.epoch);
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sh.task = atomicAdd(a.counter, 1);
+            sh.abort_at = 0x7fffffff;
+        }
+        __syncthreads();
+        const int g = __builtin_amdgcn_readfirstlane(sh.task);
+        __syncthreads();
+        if (g >= a.nstrips) break;
+        stencil_strip<OP, W>(a, g, sh, lane, wave, epoch);
+        __syncthreads();
+    }
+}
+
+}  // namespace wfs
+
+static const void *stencil_fn(int op, int waves)
+{
+    if (op == WFS_SAT) {
+        if (waves == 1) return reinterpret_cast<const void *>(&wfs::wf_stencil_kernel<WFS_SAT, 1>);
+        if (waves == 2) return reinterpret_cast<const void *>(&wfs::wf_stencil_kernel<WFS_SAT, 2>);
+        if (waves == 4) return reinterpret_cast<const void *>(&wfs::wf_stencil_kernel<WFS_SAT, 4>);
+    } else if (op == WFS_SOR) {
+        if (waves == 1) return reinterpret_cast<const void *>(&wfs::wf_stencil_kernel<WFS_SOR, 1>);
+        if (waves == 2) return reinterpret_cast<const void *>(&wfs::wf_stencil_kernel<WFS_SOR, 2>);
+        if (waves == 4) return reinterpret_cast<const void *>(&wfs::wf_stencil_kernel<WFS_SOR, 4>);
+    }
+    return nullptr;
+}
+
+extern "C" int wf_stencil_resident(int op, int waves, int cus)
+{
+    const void *fn = stencil_fn(op, waves);
+    if (!fn) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * waves, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    return per_cu * cus;
+}
+
+extern "C" hipError_t wf_launch_stencil(int op, int waves, int grid, const WfStencilArgs *a, hipStream_t stream)
+{
+    const void *fn = stencil_fn(op, waves);
+    if (!fn || grid <= 0) return hipErrorInvalidValue;
+    void *args[] = {const_cast<WfStencilArgs *>(a)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(64 * waves), args, 0, stream);
+}

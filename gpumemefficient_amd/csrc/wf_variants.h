@@ -34,4 +34,9 @@ extern "C" hipError_t wf_launch_naive(int method, int prob, int textbook, int *t
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
 
+// Stencil sweep (wf_stencil.hip): op WFS_SAT / WFS_SOR, waves 1, 2 or 4 per
+// workgroup; grid <= 0 picks one workgroup per strip up to the resident limit.
+extern "C" hipError_t wf_launch_stencil(int op, int waves, int grid, const WfStencilArgs *a, hipStream_t stream);
+extern "C" int wf_stencil_resident(int op, int waves, int cus);
+
 #endif

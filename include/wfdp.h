@@ -76,6 +76,8 @@ typedef struct {
  * 512-row strips per CU) and the lock-step engine otherwise. */
 #define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine */
 #define WF_TILE_ASYNC 0x8       /* asynchronous engine: LDS progress counters + I/O wave */
+#define WF_TILE_NO_COPYBACK 0x10 /* wf_stencil: leave the host table unchanged (the
+                                    reference never copies its result back) */
 
 typedef struct {
     int32_t last;      /* bottom-right cell: the score the reference prints */
@@ -151,6 +153,38 @@ int wf_align_table(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const
 #define WF_NAIVE_TILE 1
 int wf_align_naive(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
                    int64_t n2, const wf_tile *tile, int method, int32_t *table, wf_result *out);
+
+/* In-place table stencils swept with the same wavefront schedule: the
+ * reference's other users of the hyperlane engine, whose host function is
+ *   void SOR(int n1, int n2, int *table, int tX, int tY);
+ *       SAT/GPU_Hyperlane_Share.h:5, 2D-SOR/nested_loop/GPU_Hyperlane_Share.h:5
+ *   void SOR(int n1, int n2, int *table);
+ *       SAT/GPU_Hyperlane_Cache.h:5, 2D-SOR/nested_loop/GPU_Hyperlane_Cache.h:5
+ *   WF_SAT  summed-area table: T(y,x) = T(y,x-1) + T(y-1,x) + T(y,x) - T(y-1,x-1)
+ *           (SAT/GPU_Hyperlane_Share.cu:169), int32 arithmetic wrapping mod 2^32.
+ *   WF_SOR  one Gauss-Seidel sweep: T(y,x) = (T(y,x-1) + T(y-1,x) + T(y,x) +
+ *           T(y,x+1) + T(y+1,x)) / 5 with left and up already updated
+ *           (2D-SOR/nested_loop/GPU_Hyperlane_Share.cu:169); the sum wraps mod
+ *           2^32, the division truncates toward zero.
+ * The table is the reference's: n2 + 2 rows of pitch ld >= n1 + 3 int32
+ * (rowsize = n1 + 3, colsize = n2 + 2, SAT/GPU_Hyperlane_Share.cu:261-263).
+ * Row 0, row n2 + 1 and columns 0, n1 + 1, n1 + 2 are boundary values; the
+ * interior 1 <= y <= n2, 1 <= x <= n1 is updated in place, as the row-major
+ * sequential sweep would update it.  The reference computes on a device copy
+ * that it never copies back (SAT/GPU_Hyperlane_Share.cu:317); these functions
+ * return the result (wf_stencil: unless WF_TILE_NO_COPYBACK).  tile:
+ * strip_rows 64, 128 or 256 (0 = choose); rows_per_lane 0 or 1.  out->last =
+ * the updated cell (n2, n1). */
+typedef enum { WF_SAT = 0, WF_SOR = 1 } wf_stencil_op;
+/* Host table, updated in place (uploaded, swept, downloaded).  WF_ERR_NOMEM if
+ * the table does not fit in free HBM. */
+int wf_stencil(wf_ctx *ctx, int op, int64_t n1, int64_t n2, int32_t *table, int64_t ld,
+               const wf_tile *tile, wf_result *out);
+/* Device-resident table (caller-owned device memory): ld % 4 == 0 and a
+ * 16-byte aligned base, else WF_ERR_INVALID.  Synchronous on the context's
+ * stream; out->kernel_ms is the sweep's device time. */
+int wf_stencil_device(wf_ctx *ctx, int op, int64_t n1, int64_t n2, int32_t *dev_table, int64_t ld,
+                      const wf_tile *tile, wf_result *out);
 
 int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                    wf_result *out);

@@ -45,6 +45,16 @@ int SW(int n1, int n2, int *arr1, int *arr2);
 /* EditDistance/GPU.h:5 (textbook row 0, EditDistance/GPU.cu:12-18) */
 int ED(int n1, int n2, int *arr1, int *arr2);
 
+/* The SAT and 2D-SOR programs' host function, named SOR in both
+ * (libwfdp_compat_sat.so: summed-area table; libwfdp_compat_sor.so: one
+ * Gauss-Seidel sweep).  `table` holds (n2 + 2) rows of n1 + 3 values; like
+ * the reference it is left unchanged unless WFDP_DEBUG_TABLE=1, which writes
+ * the swept table back. */
+/* SAT/GPU_Hyperlane_Share.h:5, 2D-SOR/nested_loop/GPU_Hyperlane_Share.h:5 */
+void SOR(int n1, int n2, int *table, int tX, int tY);
+/* SAT/GPU_Hyperlane_Cache.h:5, 2D-SOR/nested_loop/GPU_Hyperlane_Cache.h:5 */
+void SOR(int n1, int n2, int *table);
+
 /* Extension used by the drop-in LCS driver's optional 5-argument form. */
 int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
 

@@ -12,6 +12,11 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   ed   golden vectors from ED.cpp (including the row-0 quirk), brute force of
        the textbook Levenshtein distance in textbook mode, closed forms
   table / last_row  checked against lcs/sw/ed and the reference full tables
+  stencil  SAT: the closed form (2-D prefix sums of the interior plus the
+       boundary row and column, mod 2^32); SOR: hand-computed known answers
+       (which neighbours are new, which old; truncation of negative sums),
+       constant tables as fixed points, and an anti-diagonal traversal that
+       must give the row-major result
 """
 from __future__ import annotations
 
@@ -51,6 +56,8 @@ def load() -> C.CDLL:
     lib.oracle_table.restype = C.c_int32
     lib.oracle_last_row.argtypes = [C.c_int, I32P, C.c_int64, I32P, C.c_int64, C.c_int, I32P]
     lib.oracle_last_row.restype = C.c_int
+    lib.oracle_stencil.argtypes = [C.c_int, C.c_int64, C.c_int64, I32P, C.c_int64]
+    lib.oracle_stencil.restype = None
     _lib = lib
     return lib
 
@@ -107,3 +114,20 @@ def last_row(problem, a, b, textbook: bool = False) -> np.ndarray:
     out = np.zeros(a.size + 1, dtype=np.int32)
     load().oracle_last_row(problem_id(problem), _p(a), a.size, _p(b), b.size, 1 if textbook else 0, _p(out))
     return out
+
+
+SAT, SOR = 0, 1
+
+
+def stencil(op, table: np.ndarray) -> np.ndarray:
+    """One in-place sweep (op "SAT"/"SOR" or 0/1) of an (n2+2) x ld int32
+    table with ld >= n1+3; returns the updated copy.  n1 = ld - 3 unless
+    the table carries more columns: pass a view of shape (n2+2, n1+3)."""
+    op = {"SAT": SAT, "SOR": SOR}[op.upper()] if isinstance(op, str) else int(op)
+    t = np.array(table, dtype=np.int32, copy=True, order="C")
+    n2 = t.shape[0] - 2
+    n1 = t.shape[1] - 3
+    if n1 < 0 or n2 < 0:
+        raise ValueError("table must have at least 2 rows and 3 columns")
+    load().oracle_stencil(op, n1, n2, _p(t), t.shape[1])
+    return t

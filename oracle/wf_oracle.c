@@ -218,3 +218,41 @@ void oracle_batch(const int32_t *problems, const int32_t *const *as,
         if (maxs) maxs[i] = m;
     }
 }
+
+/*
+ * In-place table stencils: the reference's other users of the hyperlane
+ * schedule.  The table has n2 + 2 rows of pitch ld >= n1 + 3 (rowsize = n1+3,
+ * colsize = n2+2: SAT/GPU_Hyperlane_Share.cu:261-263); row 0, row n2+1 and
+ * columns 0, n1+1, n1+2 are read-only boundary values and the interior
+ * 1 <= y <= n2, 1 <= x <= n1 is updated in row-major order.
+ *
+ *   SAT  T(y,x) = T(y,x-1) + T(y-1,x) + T(y,x) - T(y-1,x-1)
+ *        (SAT/GPU_Hyperlane_Share.cu:169, :196, :233)
+ *   SOR  T(y,x) = (T(y,x-1) + T(y-1,x) + T(y,x) + T(y,x+1) + T(y+1,x)) / 5
+ *        (2D-SOR/nested_loop/GPU_Hyperlane_Share.cu:169, :197, :233)
+ *
+ * The reference's hyperlane tiles visit every cell after its left and upper
+ * neighbours and before its right and lower ones (the strip below lags by
+ * YoverX + 1 tiles, SAT/GPU_Hyperlane_Share.cu:106-112), which is the
+ * row-major order's dependence pattern, so this sequential sweep is its
+ * result.  The reference computes in C int; here sums wrap mod 2^32 (the
+ * reference's overflow is undefined behaviour, two's complement wrap on the
+ * GPU) and SOR's division truncates toward zero as C's does.
+ */
+void oracle_stencil(int op, int64_t n1, int64_t n2, int32_t *t, int64_t ld)
+{
+    for (int64_t y = 1; y <= n2; ++y) {
+        int32_t *row = t + y * ld;
+        const int32_t *up = row - ld;
+        const int32_t *dn = row + ld;
+        for (int64_t x = 1; x <= n1; ++x) {
+            if (op == ORACLE_SAT) {
+                row[x] = (int32_t)((uint32_t)row[x - 1] + (uint32_t)up[x] + (uint32_t)row[x] - (uint32_t)up[x - 1]);
+            } else {
+                const int32_t s = (int32_t)((uint32_t)row[x - 1] + (uint32_t)up[x] + (uint32_t)row[x] +
+                                            (uint32_t)row[x + 1] + (uint32_t)dn[x]);
+                row[x] = s / 5;
+            }
+        }
+    }
+}

@@ -20,6 +20,10 @@ int32_t oracle_table(int problem, const int32_t *a, int64_t n1,
                      int32_t *table);
 int oracle_last_row(int problem, const int32_t *a, int64_t n1,
                     const int32_t *b, int64_t n2, int textbook, int32_t *out);
+/* In-place table stencils (SAT/GPU_Hyperlane_Share.cu,
+ * 2D-SOR/nested_loop/GPU_Hyperlane_Share.cu): op 0 = SAT, 1 = SOR. */
+enum { ORACLE_SAT = 0, ORACLE_SOR = 1 };
+void oracle_stencil(int op, int64_t n1, int64_t n2, int32_t *t, int64_t ld);
 void oracle_batch(const int32_t *problems, const int32_t *const *as,
                   const int64_t *n1s, const int32_t *const *bs,
                   const int64_t *n2s, int64_t npairs, int32_t *results,

@@ -32,8 +32,10 @@ DRIVERS = [
     ("LCS-SHFL", "GPU_Hyperlane_Share"), ("LCS-SHFL", "GPU_Hyperlane_Cache"),
     ("LCS", "GPU"), ("SmithWaterman", "GPU"), ("EditDistance", "GPU"),
     ("LCS", "GPU_TILE_SHARE"), ("SmithWaterman", "GPU_TILE_SHARE"), ("EditDistance", "GPU_TILE_SHARE"),
+    ("SAT", "GPU_Hyperlane_Share"), ("SAT", "GPU_Hyperlane_Cache"),
+    ("2D-SOR", "GPU_Hyperlane_Share"), ("2D-SOR", "GPU_Hyperlane_Cache"),
 ]
-HYPERLANE = [d for d in DRIVERS if d[1].startswith("GPU_Hyperlane")]
+HYPERLANE = [d for d in DRIVERS if d[1].startswith("GPU_Hyperlane") and d[0] not in ("SAT", "2D-SOR")]
 
 
 def _header_functions(path: Path) -> set[str]:
@@ -69,6 +71,14 @@ def test_compat_libraries_export_reference_signatures():
                              check=True).stdout
         for sym in syms:
             assert sym in out, f"{p.name} lacks {sym}"
+    # SOR(int, int, int*, int, int) and SOR(int, int, int*) of the SAT and
+    # 2D-SOR programs (SAT/GPU_Hyperlane_Share.h:5, SAT/GPU_Hyperlane_Cache.h:5).
+    for policy in ("sat", "sor"):
+        p = LIBDIR / f"libwfdp_compat_{policy}.so"
+        out = subprocess.run(["nm", "-D", "--defined-only", str(p)], capture_output=True, text=True,
+                             check=True).stdout
+        for sym in ("_Z3SORiiPiii", "_Z3SORiiPi"):
+            assert sym in out, f"{p.name} lacks {sym}"
 
 
 def test_invalid_arguments_rejected_without_device():
@@ -78,6 +88,8 @@ def test_invalid_arguments_rejected_without_device():
     assert lib.wf_align(None, 0, None, 0, None, 0, None, ctypes.byref(out)) == _lib.WF_ERR_INVALID
     assert lib.wf_plan_run(None, 0) == _lib.WF_ERR_INVALID
     assert lib.wf_ctx_set_timeout(None, 1.0) == _lib.WF_ERR_INVALID
+    assert lib.wf_stencil(None, 0, 1, 1, None, 4, None, ctypes.byref(out)) == _lib.WF_ERR_INVALID
+    assert lib.wf_stencil_device(None, 1, 1, 1, None, 4, None, ctypes.byref(out)) == _lib.WF_ERR_INVALID
 
 
 @pytest.mark.parametrize("pdir,name", DRIVERS)
@@ -86,8 +98,8 @@ def test_driver_argv_errors(tmp_path, pdir, name):
     r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 1
     assert r.stdout.startswith("Incorrect Input Parameters.")
-    args = ["9", "9", "32", "128", "1"] if (name.endswith("Share") and pdir in ("SmithWaterman", "EditDistance")) \
-        else ["9", "9"]
+    args = ["9", "9", "32", "128", "1"] if (name.endswith("Share") and pdir in ("SmithWaterman", "EditDistance",
+                                                                               "SAT", "2D-SOR")) else ["9", "9"]
     r = subprocess.run([str(exe), *args], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 1
     assert "ERROR: Input file cannot be opened!" in r.stdout
@@ -165,3 +177,26 @@ def test_lcs_driver_published_2_15(tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout
     assert r.stdout.splitlines()[1] == "lcs length: 21469"
+
+
+@pytest.mark.parametrize("prog", ["SAT", "2D-SOR"])
+def test_stencil_generator_format(tmp_path, prog):
+    """generator <e1> <e2> [seed] writes the reference's x_2_<e>_y_2_<f>.txt
+    family (SAT/generator.cpp:43-66): "n1 n2", then n2 + 3 rows of n1 + 3
+    values in [0, 10000), each smaller file the top-left corner of the larger."""
+    exe = BIN / prog / "generator"
+    r = subprocess.run([str(exe), "14", "13", "7"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    files = sorted((tmp_path / "Data").iterdir())
+    assert [f.name for f in files] == ["x_2_13_y_2_12.txt", "x_2_14_y_2_13.txt"]
+    big = (tmp_path / "Data" / "x_2_14_y_2_13.txt").read_text().split("\n")
+    small = (tmp_path / "Data" / "x_2_13_y_2_12.txt").read_text().split("\n")
+    assert big[0] == "16384 8192" and small[0] == "8192 4096"
+    rows = [ln for ln in big[1:] if ln]
+    assert len(rows) == 8192 + 3
+    first = [int(v) for v in rows[0].split()]
+    assert len(first) == 16384 + 3 and all(0 <= v < 10000 for v in first)
+    assert rows[0].endswith(" ")
+    assert [int(v) for v in small[1].split()] == first[: 8192 + 3]
+    r2 = subprocess.run([str(exe), "13", "13", "7"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r2.returncode == 0

@@ -237,3 +237,83 @@ def test_planted_mistake_is_caught():
     quirk = [c for c in VEC["cases"] if "quirk" in c.get("note", "")]
     assert quirk
     assert any(O.ed(c["a"], c["b"], textbook=True) != c["ed"] for c in quirk)
+
+
+# ----------------------------------------------------------------------------
+# In-place table stencils (SAT/GPU_Hyperlane_Share.cu:169,
+# 2D-SOR/nested_loop/GPU_Hyperlane_Share.cu:169): pinned by a closed form
+# (SAT), hand-computed known answers and traversal-order invariance (SOR).
+# ----------------------------------------------------------------------------
+
+def _st_table(n1, n2, seed, lo=0, hi=10000):
+    rng = np.random.default_rng(seed)
+    return rng.integers(lo, hi, size=(n2 + 2, n1 + 3), dtype=np.int64).astype(np.int32)
+
+
+@pytest.mark.parametrize("n1,n2,lo,hi", [(1, 1, 0, 10000), (7, 5, 0, 10000), (60, 45, 0, 10000),
+                                         (33, 70, -(2 ** 31), 2 ** 31)])
+def test_sat_closed_form(n1, n2, lo, hi):
+    """T(y,x) = sum of the interior inputs above-left of (y,x) plus the
+    boundary: T(y,0) + T(0,x) - T(0,0), all mod 2^32 -- independent of the
+    recurrence (a wrong sign of the diagonal, or an old/new mix-up, breaks it)."""
+    t = _st_table(n1, n2, n1 * 31 + n2, lo, hi)
+    got = O.stencil("SAT", t)
+    v = t.astype(np.int64)
+    inner = np.zeros((n2 + 1, n1 + 1), np.int64)
+    inner[1:, 1:] = v[1:n2 + 1, 1:n1 + 1].cumsum(0).cumsum(1)
+    want = inner + v[: n2 + 1, :1] + v[:1, : n1 + 1] - v[0, 0]
+    want = ((want + 2 ** 31) % 2 ** 32 - 2 ** 31).astype(np.int32)
+    assert np.array_equal(got[: n2 + 1, : n1 + 1], want)
+    # Boundary row n2+1 and columns n1+1, n1+2 are untouched.
+    assert np.array_equal(got[n2 + 1], t[n2 + 1])
+    assert np.array_equal(got[:, n1 + 1:], t[:, n1 + 1:])
+
+
+def test_sor_known_answers():
+    """A 2 x 2 interior worked by hand: (1,1) sees the old right and down
+    values, (1,2) the new left, (2,1) the new up, (2,2) both new."""
+    t = np.array([[0, 10, 20, 0, 0],
+                  [30, 40, 50, 60, 0],
+                  [70, 80, 90, 100, 0],
+                  [0, 110, 120, 0, 0]], dtype=np.int32)
+    got = O.stencil("SOR", t)
+    a = (30 + 10 + 40 + 50 + 80) // 5          # 42
+    b = (a + 20 + 50 + 60 + 90) // 5            # 52
+    c = (70 + a + 80 + 90 + 110) // 5           # 78
+    d = (c + b + 90 + 100 + 120) // 5           # 88
+    assert (a, b, c, d) == (42, 52, 78, 88)
+    assert got[1, 1:3].tolist() == [a, b] and got[2, 1:3].tolist() == [c, d]
+    assert np.array_equal(got[0], t[0]) and np.array_equal(got[3], t[3])
+
+
+def test_sor_truncates_toward_zero():
+    t = np.zeros((3, 4), np.int32)
+    t[1, 1] = -7  # sum -7: C division gives -1, floor division would give -2
+    assert O.stencil("SOR", t)[1, 1] == -1
+    t[1, 1] = 7
+    assert O.stencil("SOR", t)[1, 1] == 1
+
+
+@pytest.mark.parametrize("c", [0, 1, -3, 9999])
+def test_sor_constant_table_is_fixed(c):
+    t = np.full((12, 15), c, np.int32)
+    assert np.array_equal(O.stencil("SOR", t), t)
+
+
+def _sor_antidiagonal(t):
+    """The same sweep in anti-diagonal order (the hyperlane tiles' order):
+    it must equal the row-major one."""
+    t = t.astype(np.int64).copy()
+    n2, n1 = t.shape[0] - 2, t.shape[1] - 3
+    for d in range(2, n1 + n2 + 1):
+        for y in range(max(1, d - n1), min(n2, d - 1) + 1):
+            x = d - y
+            s = t[y, x - 1] + t[y - 1, x] + t[y, x] + t[y, x + 1] + t[y + 1, x]
+            t[y, x] = int(s / 5)  # values stay small: no wrap, truncation toward zero
+    return t.astype(np.int32)
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 1), (6, 9), (17, 11)])
+def test_sor_traversal_order_invariant(n1, n2):
+    t = _st_table(n1, n2, n1 + 100 * n2, -5000, 5000)
+    assert np.array_equal(O.stencil("SOR", t), _sor_antidiagonal(t))
