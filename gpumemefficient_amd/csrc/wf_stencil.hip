@@ -27,7 +27,7 @@
 // strip's first wave the strip above through HBM, fills).  Old values (self,
 // right, and the row below) come from an LDS window of the wave's 64 rows plus
 // the row below, staged from HBM in 32-column blocks with coalesced 16-byte
-// loads one chunk ahead of use; each new value is written back into the
+// loads issued two chunks ahead of use; each new value is written back into the
 // window in place, and finished blocks go to HBM with coalesced 16-byte
 // stores.  Every cell is read once from and written once to HBM: the sweep
 // is bound by HBM bandwidth, not by the VALU.
@@ -73,6 +73,16 @@ struct StShared {
     int abort_at;
 };
 
+// Barrier between the waves of the workgroup that orders their LDS accesses
+// only: global loads issued before it stay in flight (a __syncthreads would
+// wait for them, and for the flushes' stores, at every chunk).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One block of the window (columns [32b, 32b+32) of the wave's 65 rows) in
 // flight between its global loads and its LDS stores: vector v = lane + 64k
 // is row v / 8, columns 32b + 4 (v % 8) .. + 3.
@@ -80,18 +90,26 @@ struct Stage {
     int4 v[(kVec + 63) / 64];
 };
 
-__device__ __forceinline__ void stage_load(Stage &sg, const WfStencilArgs &a, int yw, int b, int lane)
+// The wave's 65 window rows as a buffer resource: loads past its last row
+// (rows below the table) read zeros, so every block load is issued
+// unconditionally, with no per-lane branch (see the chunk loop).  Offsets
+// are 32-bit: 73 rows of the pitch must fit (checked by the host).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t window_rsrc(const WfStencilArgs &a, int yw)
+{
+    const int rows = max(0, min(kTR, a.n2 + 2 - yw));
+    // Third descriptor dword of an untyped buffer on the gfx9 family.
+    return __builtin_amdgcn_make_buffer_rsrc(a.t + (int64_t)yw * a.ld, 0, (int)(rows * a.ld * 4), 0x00020000);
+}
+
+__device__ __forceinline__ void stage_load(Stage &sg, __amdgpu_buffer_rsrc_t rs, int64_t ld, int b, int lane)
 {
 #pragma unroll
     for (int k = 0; k < (kVec + 63) / 64; ++k) {
         const int v = lane + 64 * k;
-        const int rr = v >> 3;
-        const int c = 32 * b + 4 * (v & 7);
-        int4 val = make_int4(0, 0, 0, 0);
-        // The pitch is a multiple of 4, so a vector is inside a row or past it.
-        if (v < kVec && yw + rr <= a.n2 + 1 && c < a.ld)
-            val = *reinterpret_cast<const int4 *>(a.t + (int64_t)(yw + rr) * a.ld + c);
-        sg.v[k] = val;
+        // Row v / 8 of the window; a vector past the pitch reads the next
+        // row's first columns (never used: the table's columns end at n1 + 2).
+        const uint32_t off = (uint32_t)(((v >> 3) * ld + 32 * b + 4 * (v & 7)) * 4);
+        sg.v[k] = __builtin_bit_cast(int4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
     }
 }
 
@@ -131,43 +149,84 @@ __device__ __forceinline__ void flush_block(const WfStencilArgs &a, const int *w
 // value (its left neighbour), `dg` the upper neighbour of the previous step
 // (SAT's diagonal).  MASKED: some lane is outside its columns 1..n1 or holds
 // no interior row; those lanes keep their state and write nothing.
+//
+// The old values a group of 4 steps needs (own row: columns x..x+4; row
+// below: x..x+3), and lane 0's upper neighbours from the ring, are read one
+// group ahead, before the current group writes
+// its new values: the reads are of columns no lane has updated yet, and the
+// LDS latency overlaps a group of arithmetic.  Reads may run past slot 127
+// into the mirror of slots 0..3 (old values, which is what they need);
+// writes always go to the slot itself.
 template <int OP, bool MASKED>
 __device__ __forceinline__ void st_chunk(int s0, int lane, int n1, bool rowok, int &h, int &dg, int *wrow,
                                          const int *ring_prev, int *ring_cur, int *dummy)
 {
+    constexpr int NO = (OP == WFS_SOR) ? 5 : 4;  // own-row values per group
+    constexpr int ND = (OP == WFS_SOR) ? 4 : 1;  // row-below values per group (SAT: none)
     const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (kRS - 1));
     int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (kRS - 1)) : dummy + lane;
     int *cap_last = (lane == 63) ? ring_cur + ((s0 + kK) & (kRS - 1)) : dummy + lane + (kK - 1);
+    const int xb = s0 + 1 - lane;  // the lane's column at step s0
+    int own[NO], dn[ND];
+    int4 inj4;
+    auto load_ops = [&](int q, int (&o)[NO], int (&d)[ND], int4 &in4) {
+        in4 = *reinterpret_cast<const int4 *>(inj_base + q);
+        const int p0 = (xb + q) & (kTW - 1);
+#pragma unroll
+        for (int u = 0; u < NO; ++u) o[u] = wrow[p0 + u];
+        if (OP == WFS_SOR) {
+#pragma unroll
+            for (int u = 0; u < ND; ++u) d[u] = wrow[kTS + p0 + u];
+        } else {
+            d[0] = 0;
+        }
+    };
+    load_ops(0, own, dn, inj4);
 #pragma unroll
     for (int q = 0; q < kK; q += 4) {
-        const int4 inj4 = *reinterpret_cast<const int4 *>(inj_base + q);
         const int injv[4] = {inj4.x, inj4.y, inj4.z, inj4.w};
+        int nown[NO], ndn[ND];
+        int4 ninj4 = inj4;
+        int capv[4];
+        if (q + 4 < kK) load_ops(q + 4, nown, ndn, ninj4);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = q + u;
-            const int x = s0 + j + 1 - lane;
-            const int p = x & (kTW - 1);
-            const int self = wrow[p];
+            const int x = xb + j;
+            const int self = own[u];
             const int up = __builtin_amdgcn_update_dpp(injv[u], h, kDppWaveShr1, 0xf, 0xf, false);
             int nv;
             if (OP == WFS_SAT) {
                 nv = (int)((uint32_t)h + (uint32_t)up + (uint32_t)self - (uint32_t)dg);
             } else {
-                const int right = wrow[p + 1];
-                const int down = wrow[p + kTS];
+                const int right = own[u + 1];
+                const int down = dn[u];
                 const int sum = (int)((uint32_t)h + (uint32_t)up + (uint32_t)self + (uint32_t)right +
                                       (uint32_t)down);
                 nv = sum / 5;
             }
             bool valid = true;
             if (MASKED) valid = rowok && (unsigned)(x - 1) < (unsigned)n1;
-            if (valid) wrow[p] = nv;
+            if (valid) wrow[x & (kTW - 1)] = nv;
             h = valid ? nv : h;
             dg = up;
+            capv[u] = h;
+        }
+        // The group's ring captures together (adjacent slots: paired writes).
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = q + u;
             if (j == kK - 1)
-                *cap_last = h;
+                *cap_last = capv[u];
             else
-                cap[j] = h;
+                cap[j] = capv[u];
+        }
+        if (q + 4 < kK) {
+            inj4 = ninj4;
+#pragma unroll
+            for (int u = 0; u < NO; ++u) own[u] = nown[u];
+#pragma unroll
+            for (int u = 0; u < ND; ++u) dn[u] = ndn[u];
         }
     }
 }
@@ -193,13 +252,15 @@ __device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, in
     const uint64_t tag = (uint64_t)epoch << 32;
     int *win = sh.win[wave];
     int *wrow = win + lane * kTS;
+    const __amdgpu_buffer_rsrc_t rs = window_rsrc(a, yw);
 
     // Top row (columns [32c, 32c+32), wave 0, lanes < 32): row 0 of the table
     // for the first strip, else the previous strip's bottom row.
     auto load_top = [&](int c) -> uint64_t {
         const int x = c * kK + lane;
         if (lane >= kK || x > n1) return tag;
-        if (!has_prev) return tag | (uint32_t)a.t[x];
+        if (!has_prev)  // row 0 is never written: any cache level may serve it
+            return tag | (uint32_t)__hip_atomic_load(a.t + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         return __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
     auto fill = [&](int c, uint64_t top) -> bool {
@@ -219,18 +280,25 @@ __device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, in
         return true;
     };
 
-    // Prologue: block 0 of the window in LDS, block 1 in flight; the top
+    // Prologue: block 0 of the window in LDS, blocks 1 and 2 in flight (block
+    // b travels in sg[b & 1], two chunks ahead of its LDS store); the top
     // ring's first two chunks.
-    Stage sg;
+    Stage sg0, sg1;
     if (active) {
-        stage_load(sg, a, yw, 0, lane);
-        stage_store(sg, win, 0, lane);
-        stage_load(sg, a, yw, 1, lane);
+        stage_load(sg0, rs, a.ld, 0, lane);
+        stage_store(sg0, win, 0, lane);
+        stage_load(sg1, rs, a.ld, 1, lane);
+        stage_load(sg0, rs, a.ld, 2, lane);
     }
+    // Top-row loads run two chunks ahead of their ring fill (top0 / top1
+    // alternate like the blocks), so an HBM round trip hides behind two
+    // chunks of arithmetic.
+    uint64_t top0 = tag, top1 = tag;
     if (wave == 0) {
         bool ok = fill(0, load_top(0));
         if (ok && nfill > 1) ok = fill(1, load_top(1));
         if (!ok && lane == 0) sh.abort_at = -1;
+        top1 = load_top(2);
     }
     __syncthreads();
 
@@ -248,35 +316,58 @@ __device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, in
         }
     };
 
+    // Every wave passes the same `total` barriers: kD idle chunks per wave
+    // above it, its nch chunks, then idle chunks until the last wave is done.
+    // The active chunks run in a loop of their own so that their nine block
+    // loads are issued on every iteration: the compiler then waits for a
+    // block's loads (two chunks old) without waiting for the newer ones.
     bool aborted = sh.abort_at <= -1;
-    for (int j = 0; j < total && !aborted; ++j) {
-        const int cj = j - wave * kD;
-        uint64_t top_pre = tag;
-        if (wave == 0 && cj + 2 < nfill) top_pre = load_top(cj + 2);
-        if (active && cj >= 0 && cj < nch) {
-            const int s0 = cj * kK;
-            // Window: block cj+1 (loaded during the previous chunk) takes the
-            // slot of block cj-3, flushed at the end of the previous chunk;
-            // block cj+2 is loaded for the next chunk.
-            stage_store(sg, win, cj + 1, lane);
-            stage_load(sg, a, yw, cj + 2, lane);
-            if (cj == 0 && lane == 0) dg = sh.ring[wave][63];  // column 0 of the row above
-            const bool steady = nrows == 64 && s0 >= 63 && s0 + kK - 1 <= n1 - 1;
-            if (steady)
-                st_chunk<OP, false>(s0, lane, n1, rowok, h, dg, wrow, sh.ring[wave], sh.ring[wave + 1],
-                                    sh.dummy[wave]);
-            else
-                st_chunk<OP, true>(s0, lane, n1, rowok, h, dg, wrow, sh.ring[wave], sh.ring[wave + 1],
-                                   sh.dummy[wave]);
-            // Steps < s0 + kK are done: lane 63 has finished columns < s0 + kK - 62,
-            // so every lane has finished block cj-2.
-            if (wave == W - 1 && has_next) copy_out(min(n1 + 1, s0 + kK - 62), true);
-            flush_block(a, win, yw, cj - 2, nrows, lane);
-        }
+    int j = 0;
+    const int jstart = active ? wave * kD : total;
+    for (; j < jstart && !aborted; ++j) {
+        lds_barrier();
+        aborted = sh.abort_at <= j;
+    }
+    // One active chunk; `sg` holds block cj+1 (loaded two chunks ago) and
+    // takes block cj+3; `top_old` holds the top row's chunk cj+2 (loaded
+    // during the previous chunk), `top_new` takes chunk cj+3.
+    auto chunk = [&](int cj, Stage &sg, uint64_t &top_new, uint64_t &top_old) {
+        const int s0 = cj * kK;
+        if (wave == 0 && cj + 3 < nfill) top_new = load_top(cj + 3);
+        // Keep the top-row load ahead of the block loads (the compiler would
+        // otherwise sink it toward its use, where waiting for it would also
+        // wait for the block loads issued after it).
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        // Window: block cj+1 takes the slot of block cj-3, flushed at the end
+        // of the previous chunk.
+        stage_store(sg, win, cj + 1, lane);
+        stage_load(sg, rs, a.ld, cj + 3, lane);
+        if (cj == 0 && lane == 0) dg = sh.ring[wave][63];  // column 0 of the row above
+        const bool steady = nrows == 64 && s0 >= 63 && s0 + kK - 1 <= n1 - 1;
+        if (steady)
+            st_chunk<OP, false>(s0, lane, n1, rowok, h, dg, wrow, sh.ring[wave], sh.ring[wave + 1], sh.dummy[wave]);
+        else
+            st_chunk<OP, true>(s0, lane, n1, rowok, h, dg, wrow, sh.ring[wave], sh.ring[wave + 1], sh.dummy[wave]);
+        // Steps < s0 + kK are done: lane 63 has finished columns < s0 + kK - 62,
+        // so every lane has finished block cj-2.
+        if (wave == W - 1 && has_next) copy_out(min(n1 + 1, s0 + kK - 62), true);
+        flush_block(a, win, yw, cj - 2, nrows, lane);
         if (wave == 0 && cj + 2 < nfill) {
-            if (!fill(cj + 2, top_pre) && lane == 0) sh.abort_at = j;
+            if (!fill(cj + 2, top_old) && lane == 0) sh.abort_at = j;
         }
-        __syncthreads();
+        lds_barrier();
+        aborted = sh.abort_at <= j;
+        ++j;
+    };
+    // Even chunks stage from sg1, odd ones from sg0; the loop takes them in
+    // pairs so that the alternation is visible to the compiler.
+    for (int cj = 0; cj < nch && !aborted; cj += 2) {
+        chunk(cj, sg1, top0, top1);
+        if (cj + 1 >= nch || aborted) break;
+        chunk(cj + 1, sg0, top1, top0);
+    }
+    for (; j < total && !aborted; ++j) {
+        lds_barrier();
         aborted = sh.abort_at <= j;
     }
 

@@ -97,7 +97,9 @@ int upload_run_download(wf_ctx *ctx, int op, int64_t n1, int64_t n2, int32_t *ta
 
 bool valid_dims(int op, int64_t n1, int64_t n2, int64_t ld)
 {
-    return (op == WF_SAT || op == WF_SOR) && n1 >= 0 && n2 >= 0 && n1 + 3 <= ld && n1 <= INT32_MAX - 256 &&
+    // The kernel addresses a wave's 65-row window (and 8 rows of overrun)
+    // with 32-bit byte offsets.
+    return (op == WF_SAT || op == WF_SOR) && n1 >= 0 && n2 >= 0 && n1 + 3 <= ld && ld <= (int64_t)WF_STENCIL_MAX_PITCH &&
            n2 <= INT32_MAX - 256;
 }
 

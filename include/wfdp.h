@@ -176,6 +176,7 @@ int wf_align_naive(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const
  * strip_rows 64, 128 or 256 (0 = choose); rows_per_lane 0 or 1.  out->last =
  * the updated cell (n2, n1). */
 typedef enum { WF_SAT = 0, WF_SOR = 1 } wf_stencil_op;
+#define WF_STENCIL_MAX_PITCH 14000000 /* largest ld (and n1 + 3) accepted */
 /* Host table, updated in place (uploaded, swept, downloaded).  WF_ERR_NOMEM if
  * the table does not fit in free HBM. */
 int wf_stencil(wf_ctx *ctx, int op, int64_t n1, int64_t n2, int32_t *table, int64_t ld,
