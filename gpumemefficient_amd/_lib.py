@@ -28,6 +28,9 @@ WF_TILE_ED_TEXTBOOK = 0x1
 WF_TILE_CACHE = 0x2
 WF_TILE_LOCKSTEP = 0x4
 WF_TILE_ASYNC = 0x8
+WF_TILE_NO_COPYBACK = 0x10
+WF_TILE_BITPAR = 0x20
+ENGINES = ("lockstep", "async", "bitpar")
 
 WF_TABLE_MAX_CELLS = 1 << 28
 WF_NAIVE_DIAGONAL = 0

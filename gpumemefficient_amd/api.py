@@ -75,17 +75,19 @@ class Tile:
     rows_per_lane: int = 0
     ed_textbook: bool = False
     cache: bool = False
-    engine: str = ""      # "lockstep", "async" or "" (default)
+    engine: str = ""      # "lockstep", "async", "bitpar" or "" (default)
 
     def _c(self) -> _lib.wf_tile:
         flags = ((_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
                  | (_lib.WF_TILE_LOCKSTEP if self.engine == "lockstep" else 0)
-                 | (_lib.WF_TILE_ASYNC if self.engine == "async" else 0))
+                 | (_lib.WF_TILE_ASYNC if self.engine == "async" else 0)
+                 | (_lib.WF_TILE_BITPAR if self.engine == "bitpar" else 0))
         return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
 
     @classmethod
     def _from(cls, t: _lib.wf_tile) -> "Tile":
-        eng = "lockstep" if t.flags & _lib.WF_TILE_LOCKSTEP else ("async" if t.flags & _lib.WF_TILE_ASYNC else "")
+        eng = ("lockstep" if t.flags & _lib.WF_TILE_LOCKSTEP else "async" if t.flags & _lib.WF_TILE_ASYNC
+               else "bitpar" if t.flags & _lib.WF_TILE_BITPAR else "")
         return cls(t.tile_w, t.strip_rows, t.rows_per_lane, bool(t.flags & _lib.WF_TILE_ED_TEXTBOOK),
                    bool(t.flags & _lib.WF_TILE_CACHE), eng)
 
@@ -152,7 +154,7 @@ class Plan:
         nt = C.c_int64()
         check(self.ctx._lib.wf_plan_describe(self._h, C.byref(e), C.byref(r), C.byref(w), C.byref(k), C.byref(nt),
                                              C.byref(g)))
-        return {"engine": ("lockstep", "async")[e.value], "rows_per_lane": r.value, "waves": w.value,
+        return {"engine": _lib.ENGINES[e.value], "rows_per_lane": r.value, "waves": w.value,
                 "chunk": k.value, "tasks": nt.value, "grid": g.value}
 
     def close(self) -> None:
@@ -289,7 +291,7 @@ def variants() -> list[dict]:
     for i in range(lib.wf_variant_count()):
         e, r, w, k = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         check(lib.wf_variant_info(i, C.byref(e), C.byref(r), C.byref(w), C.byref(k)))
-        out.append({"engine": ("lockstep", "async")[e.value], "rows_per_lane": r.value, "waves": w.value,
+        out.append({"engine": _lib.ENGINES[e.value], "rows_per_lane": r.value, "waves": w.value,
                     "chunk": k.value})
     return out
 

@@ -125,12 +125,34 @@ static int32_t empty_score(int problem, int64_t n1, int64_t n2, int flags)
 // Variant choice
 // ----------------------------------------------------------------------------
 
-static int variant_count() { return wf_num_variants_async + wf_num_variants_lockstep; }
+static int variant_count() { return wf_num_variants_async + wf_num_variants_lockstep + wf_num_variants_bitpar; }
 
 static const WfVariant *variant_at(int i)
 {
     if (i < 0 || i >= variant_count()) return nullptr;
-    return i < wf_num_variants_async ? &wf_variants_async[i] : &wf_variants_lockstep[i - wf_num_variants_async];
+    if (i < wf_num_variants_async) return &wf_variants_async[i];
+    i -= wf_num_variants_async;
+    if (i < wf_num_variants_lockstep) return &wf_variants_lockstep[i];
+    return &wf_variants_bitpar[i - wf_num_variants_lockstep];
+}
+
+// The bit-parallel engine (wf_bitpar.hip) takes LCS and ED pairs whose
+// symbols map to codes below 32 (its match-mask table has 32 rows).
+static bool bitpar_eligible(const wf_job *jobs, int64_t njobs)
+{
+    for (int64_t i = 0; i < njobs; ++i) {
+        const wf_job &j = jobs[i];
+        if (j.problem != WF_LCS && j.problem != WF_ED) return false;
+        bool small = true;
+        for (int64_t k = 0; small && k < j.n1; ++k) small = (uint32_t)j.a[k] < 32u;
+        for (int64_t k = 0; small && k < j.n2; ++k) small = (uint32_t)j.b[k] < 32u;
+        if (small) continue;
+        std::unordered_map<int32_t, int> seen;
+        for (int64_t k = 0; k < j.n1 && seen.size() <= 32; ++k) seen.emplace(j.a[k], 0);
+        for (int64_t k = 0; k < j.n2 && seen.size() <= 32; ++k) seen.emplace(j.b[k], 0);
+        if (seen.size() > 32) return false;
+    }
+    return true;
 }
 
 static const WfVariant *find_variant(int E, int R, int W, int K)
@@ -167,6 +189,34 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
             if (const WfVariant *v = find_variant(te, tr, tw, tk)) return v;
         }
     }
+    // LCS and ED pairs with at most 32 symbols go to the bit-parallel engine
+    // (32 rows per lane word) unless a cell engine is asked for: its chain of
+    // waves down a pair is 32 times shorter and a column step costs about
+    // 20 VALU operations for 32 cells.
+    const bool cell_flags = tile && (tile->flags & (WF_TILE_LOCKSTEP | WF_TILE_ASYNC | WF_TILE_CACHE));
+    const bool want_bitpar = (tile && ((tile->flags & WF_TILE_BITPAR) || tile->rows_per_lane == 32)) ||
+                             (!cell_flags && !(tile && tile->rows_per_lane > 0));
+    if (want_bitpar && bitpar_eligible(jobs, njobs)) {
+        int Wb = 0, Kb = 64;
+        if (tile && tile->strip_rows > 0) Wb = std::max(1, tile->strip_rows / 2048);
+        // Two waves (4096 rows) per strip and 64-step chunks: measured best
+        // for ED 65536^2 (3.23 ms; W = 1 3.68, W = 4 3.32; K = 128 3.39)
+        // and LCS 8192^2.
+        if (Wb == 0) Wb = max_rows > 2048 ? 2 : 1;
+        if (tile && tile->tile_w > 0) Kb = tile->tile_w;
+        const WfVariant *best = nullptr;
+        int best_d = 1 << 30;
+        for (int i = 0; i < wf_num_variants_bitpar; ++i) {
+            const WfVariant &v = wf_variants_bitpar[i];
+            const int d = std::abs(v.waves - Wb) * 1000 + std::abs(v.chunk - Kb);
+            if (d < best_d) {
+                best_d = d;
+                best = &v;
+            }
+        }
+        return best;
+    }
+    if (tile && (tile->flags & WF_TILE_BITPAR)) return nullptr;  // asked for, not possible
     // Default engine: a launch whose strips all fit on the chip at once is
     // bound by the wavefront's critical path, where the asynchronous engine's
     // shorter lag between waves and strips wins; a launch with many more
@@ -312,6 +362,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if (j.problem == WF_ED && (j.flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
         if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
         p.mode = mode;
+        // The bit-parallel kernel indexes a 32-row table with the codes.
+        if (v->engine == WF_ENGINE_BITPAR && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;
     }
 
     // Tasks: pairs with the longest dependency chain first (columns plus the

@@ -130,7 +130,7 @@ int wf_tune_save(const char *path)
     FILE *f = std::fopen(path, "w");
     if (!f) return WF_ERR_INVALID;
     std::fprintf(f, "# wfdp autotune table: arch cus problem(0 LCS,1 ED,2 SW) class bucket "
-                    "engine(0 lockstep,1 async) rows_per_lane waves chunk ms\n");
+                    "engine(0 lockstep,1 async,2 bitpar) rows_per_lane waves chunk ms\n");
     std::lock_guard<std::mutex> lk(g_mu);
     for (const TuneEntry &e : g_table)
         std::fprintf(f, "%s %d %d %s %d %d %d %d %d %.4f\n", e.arch.c_str(), e.cus, e.problem,
@@ -146,6 +146,11 @@ int wf_tune_clear(void)
     return WF_OK;
 }
 
+static int32_t engine_flag(int engine)
+{
+    return engine == WF_ENGINE_LOCKSTEP ? WF_TILE_LOCKSTEP : engine == WF_ENGINE_ASYNC ? WF_TILE_ASYNC : WF_TILE_BITPAR;
+}
+
 int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_tile *best, double *best_ms)
 {
     if (!ctx || !jobs || njobs <= 0 || !best) return WF_ERR_INVALID;
@@ -159,7 +164,7 @@ int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_til
     for (int i = 0; i < nv; ++i) {
         int32_t e, R, W, K;
         wf_variant_info(i, &e, &R, &W, &K);
-        wf_tile t{K, 64 * W * R, R, e == WF_ENGINE_LOCKSTEP ? WF_TILE_LOCKSTEP : WF_TILE_ASYNC};
+        wf_tile t{K, 64 * W * R, R, engine_flag(e)};
         wf_plan *p = nullptr;
         if (wf_plan_create(ctx, jobs, njobs, &t, &p) != WF_OK) continue;
         double ms = 1e30;
@@ -177,7 +182,7 @@ int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_til
     if (bi < 0) return WF_ERR_NOKERNEL;
     int32_t e, R, W, K;
     wf_variant_info(bi, &e, &R, &W, &K);
-    *best = wf_tile{K, 64 * W * R, R, e == WF_ENGINE_LOCKSTEP ? WF_TILE_LOCKSTEP : WF_TILE_ASYNC};
+    *best = wf_tile{K, 64 * W * R, R, engine_flag(e)};
     if (best_ms) *best_ms = bms;
     TuneEntry n;
     n.arch = info.arch;

@@ -7,7 +7,9 @@
 
 #include "wf_device.h"
 
-enum { WF_ENGINE_LOCKSTEP = 0, WF_ENGINE_ASYNC = 1 };
+// WF_ENGINE_BITPAR: LCS and ED only, 32 rows per lane as one bit-vector word
+// (wf_bitpar.hip).
+enum { WF_ENGINE_LOCKSTEP = 0, WF_ENGINE_ASYNC = 1, WF_ENGINE_BITPAR = 2 };
 
 struct WfVariant {
     int engine;         // WF_ENGINE_*
@@ -22,6 +24,8 @@ extern "C" const WfVariant wf_variants_lockstep[];
 extern "C" const int wf_num_variants_lockstep;
 extern "C" const WfVariant wf_variants_async[];
 extern "C" const int wf_num_variants_async;
+extern "C" const WfVariant wf_variants_bitpar[];
+extern "C" const int wf_num_variants_bitpar;
 
 extern "C" hipError_t wf_launch_reset(WfResultDev *res, int64_t nres, int32_t *counters, int64_t ncounters,
                                       uint32_t *epoch, hipStream_t stream);

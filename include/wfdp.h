@@ -71,11 +71,15 @@ typedef struct {
 #define WF_TILE_ED_TEXTBOOK 0x1 /* ED row 0 = x (reference GPU_setInit), not ED.cpp's x-1 */
 #define WF_TILE_CACHE 0x2       /* hyperlane-Cache policy: one wave per workgroup, all
                                    strip boundaries through L2 (GPU_Hyperlane_Cache.cu) */
-/* Engine choice; without either flag the planner takes the asynchronous
- * engine when all strips of the launch fit on the chip at once (at most two
- * 512-row strips per CU) and the lock-step engine otherwise. */
+/* Engine choice; without a flag the planner takes the bit-parallel engine
+ * for LCS and ED pairs of at most 32 symbols, else the asynchronous engine
+ * when all strips of the launch fit on the chip at once (at most two 512-row
+ * strips per CU) and the lock-step engine otherwise. */
 #define WF_TILE_LOCKSTEP 0x4    /* barrier-synchronised engine */
 #define WF_TILE_ASYNC 0x8       /* asynchronous engine: LDS progress counters + I/O wave */
+#define WF_TILE_BITPAR 0x20     /* bit-parallel engine: LCS and ED with at most 32 symbols,
+                                   32 rows per lane as one bit-vector word; the default for
+                                   such pairs unless a cell engine flag is given */
 #define WF_TILE_NO_COPYBACK 0x10 /* wf_stencil: leave the host table unchanged (the
                                     reference never copies its result back) */
 
@@ -214,7 +218,8 @@ int wf_tune_load(const char *path);
 int wf_tune_save(const char *path);
 int wf_tune_clear(void);
 
-/* Compiled kernel variants (engine 0 = lock-step, 1 = asynchronous). */
+/* Compiled kernel variants (engine 0 = lock-step, 1 = asynchronous, 2 =
+ * bit-parallel, whose rows_per_lane is the 32 bits of a lane's word). */
 int wf_variant_count(void);
 int wf_variant_info(int index, int32_t *engine, int32_t *rows_per_lane, int32_t *waves,
                     int32_t *chunk);

@@ -44,6 +44,8 @@ def test_non_square(ctx, prob):
 @pytest.mark.parametrize("v", VARIANTS, ids=lambda v: f"{v['engine']}-R{v['rows_per_lane']}W{v['waves']}K{v['chunk']}")
 @pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
 def test_every_variant(ctx, prob, v):
+    if v["engine"] == "bitpar" and prob == "SW":
+        pytest.skip("the bit-parallel engine takes LCS and ED only (SW falls back to a cell engine)")
     R, W, K = v["rows_per_lane"], v["waves"], v["chunk"]
     rng = np.random.default_rng(100 + R * 31 + W * 7 + K)
     t = tile_for(v)
@@ -74,6 +76,38 @@ def test_arbitrary_symbols(ctx, prob):
     a = syms[rng.integers(0, syms.size, 800)]
     b = syms[rng.integers(0, syms.size, 600)]
     check(ctx, prob, a, b)
+
+
+BITPAR = Tile(engine="bitpar")
+
+
+@pytest.mark.parametrize("prob,textbook", [("LCS", False), ("ED", False), ("ED", True)])
+def test_bitpar_word_and_wave_edges(ctx, prob, textbook):
+    """The bit-parallel engine at the edges of its geometry: rows around one
+    32-row word, one 64-lane wave (2048 rows) and a 4-wave strip, columns
+    around one wave's 64-step skew, 2- to 32-letter alphabets."""
+    rng = np.random.default_rng(31)
+    for n1, n2, alpha in [(1, 1, 4), (2, 31, 2), (3, 32, 4), (64, 33, 4), (63, 2047, 20), (65, 2048, 32),
+                          (200, 2049, 4), (1, 8193, 4), (2500, 8192 * 2 + 3, 20), (4097, 100, 3)]:
+        a, b = rand_pair(rng, n1, n2, alpha)
+        check(ctx, prob, a, b, tile=BITPAR, textbook=textbook)
+
+
+def test_bitpar_is_the_default_for_lcs_and_ed(ctx):
+    rng = np.random.default_rng(32)
+    a, b = rand_pair(rng, 3000, 5000, 20)
+    for prob in ("LCS", "ED"):
+        plan = ctx.plan([Job(prob, a, b)])
+        assert plan.describe()["engine"] == "bitpar"
+        plan.close()
+    for prob, alpha in (("SW", 4), ("ED", 40)):  # SW, and more than 32 symbols: cell engines
+        a2, b2 = rand_pair(rng, 500, 700, alpha)
+        plan = ctx.plan([Job(prob, a2, b2)])
+        assert plan.describe()["engine"] != "bitpar"
+        plan.close()
+    from gpumemefficient_amd import WfError
+    with pytest.raises(WfError):
+        ctx.align("SW", a, b, tile=BITPAR)
 
 
 def test_ed_textbook(ctx):
