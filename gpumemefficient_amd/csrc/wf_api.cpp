@@ -138,7 +138,7 @@ static const WfVariant *variant_at(int i)
 
 // The bit-parallel engine (wf_bitpar.hip) takes LCS and ED pairs whose
 // symbols map to codes below 32 (its match-mask table has 32 rows).
-static bool bitpar_eligible(const wf_job *jobs, int64_t njobs)
+bool wf_bitpar_eligible(const wf_job *jobs, int64_t njobs)
 {
     for (int64_t i = 0; i < njobs; ++i) {
         const wf_job &j = jobs[i];
@@ -196,7 +196,7 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
     const bool cell_flags = tile && (tile->flags & (WF_TILE_LOCKSTEP | WF_TILE_ASYNC | WF_TILE_CACHE));
     const bool want_bitpar = (tile && ((tile->flags & WF_TILE_BITPAR) || tile->rows_per_lane == 32)) ||
                              (!cell_flags && !(tile && tile->rows_per_lane > 0));
-    if (want_bitpar && bitpar_eligible(jobs, njobs)) {
+    if (want_bitpar && wf_bitpar_eligible(jobs, njobs)) {
         int Wb = 0, Kb = 64;
         if (tile && tile->strip_rows > 0) Wb = std::max(1, tile->strip_rows / 2048);
         // Two waves (4096 rows) per strip and 64-step chunks: measured best

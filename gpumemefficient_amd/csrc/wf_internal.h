@@ -86,6 +86,9 @@ struct WfPlanHost {
     int grid = 0;
 };
 
+// True if every job can run on the bit-parallel engine (LCS or ED, at most
+// 32 distinct symbols).
+bool wf_bitpar_eligible(const wf_job *jobs, int64_t njobs);
 int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
                    uint8_t *out_a, uint8_t *out_b, int32_t *mode);
 const WfVariant *wf_pick_variant(struct wf_ctx *ctx, const wf_tile *tile, const wf_job *jobs, int64_t njobs,

@@ -42,6 +42,7 @@ struct Request {
     int32_t problem, flags;
     std::vector<int32_t> a, b;
     int64_t bytes;      // device footprint charged to the HBM budget
+    bool bitpar;        // can run on the bit-parallel engine (LCS/ED, <= 32 symbols)
     double t_submit;
     double t_start = 0; // launch time
 };
@@ -215,9 +216,20 @@ static void scheduler(wf_server *s)
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 const int maxb = s->opts.per_request ? 1 : std::max(1, s->opts.max_batch);
-                while (!s->admitted.empty() && (int)batch.size() < maxb) {
-                    batch.push_back(std::move(s->admitted.front()));
-                    s->admitted.pop_front();
+                // One launch runs one engine: take the oldest request and the
+                // admitted requests of its class (bit-parallel LCS/ED, or the
+                // rest), in arrival order; the other class waits for the
+                // next free slot.
+                if (!s->admitted.empty()) {
+                    const bool cls = s->admitted.front().bitpar;
+                    for (auto it = s->admitted.begin(); it != s->admitted.end() && (int)batch.size() < maxb;) {
+                        if (it->bitpar == cls) {
+                            batch.push_back(std::move(*it));
+                            it = s->admitted.erase(it);
+                        } else {
+                            ++it;
+                        }
+                    }
                 }
             }
             if (batch.empty()) break;
@@ -305,6 +317,7 @@ int wf_server_submit(wf_server *s, const wf_job *job, uint64_t *ticket)
     r.a.assign(job->a, job->a + job->n1);
     r.b.assign(job->b, job->b + job->n2);
     r.bytes = request_bytes(job->n1, job->n2, 512);
+    r.bitpar = wf_bitpar_eligible(job, 1);
     std::lock_guard<std::mutex> lk(s->mu);
     if ((int64_t)s->pending.size() >= s->opts.queue_capacity) return WF_ERR_AGAIN;
     r.ticket = s->next_ticket++;
