@@ -50,6 +50,8 @@ def peak_int32_tops(info: dict) -> float:
 def kernel_name(desc: dict) -> str:
     if desc["engine"] == "bitpar":
         return f"wf_engine_bitpar<{desc['waves']},{desc['chunk']}>"
+    if desc["engine"] == "bitpar_staged":
+        return f"wf_engine_bitpar_staged<{desc['waves']}>"
     base = "wf_engine" if desc["engine"] == "lockstep" else "wf_engine_async"
     return f"{base}<{desc['rows_per_lane']},{desc['waves']},{desc['chunk']}>"
 

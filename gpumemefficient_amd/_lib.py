@@ -30,7 +30,8 @@ WF_TILE_LOCKSTEP = 0x4
 WF_TILE_ASYNC = 0x8
 WF_TILE_NO_COPYBACK = 0x10
 WF_TILE_BITPAR = 0x20
-ENGINES = ("lockstep", "async", "bitpar")
+WF_TILE_BITPAR_STAGED = 0x40
+ENGINES = ("lockstep", "async", "bitpar", "bitpar_staged")
 
 WF_TABLE_MAX_CELLS = 1 << 28
 WF_NAIVE_DIAGONAL = 0

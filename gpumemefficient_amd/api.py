@@ -75,13 +75,14 @@ class Tile:
     rows_per_lane: int = 0
     ed_textbook: bool = False
     cache: bool = False
-    engine: str = ""      # "lockstep", "async", "bitpar" or "" (default)
+    engine: str = ""      # "lockstep", "async", "bitpar", "bitpar_staged" or "" (default)
 
     def _c(self) -> _lib.wf_tile:
         flags = ((_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
                  | (_lib.WF_TILE_LOCKSTEP if self.engine == "lockstep" else 0)
                  | (_lib.WF_TILE_ASYNC if self.engine == "async" else 0)
-                 | (_lib.WF_TILE_BITPAR if self.engine == "bitpar" else 0))
+                 | (_lib.WF_TILE_BITPAR if self.engine == "bitpar" else 0)
+                 | (_lib.WF_TILE_BITPAR_STAGED if self.engine == "bitpar_staged" else 0))
         return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
 
     @classmethod

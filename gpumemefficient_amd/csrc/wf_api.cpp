@@ -194,20 +194,37 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
     // waves down a pair is 32 times shorter and a column step costs about
     // 20 VALU operations for 32 cells.
     const bool cell_flags = tile && (tile->flags & (WF_TILE_LOCKSTEP | WF_TILE_ASYNC | WF_TILE_CACHE));
-    const bool want_bitpar = (tile && ((tile->flags & WF_TILE_BITPAR) || tile->rows_per_lane == 32)) ||
+    const int32_t bp_flags = WF_TILE_BITPAR | WF_TILE_BITPAR_STAGED;
+    const bool want_bitpar = (tile && ((tile->flags & bp_flags) || tile->rows_per_lane == 32)) ||
                              (!cell_flags && !(tile && tile->rows_per_lane > 0));
     if (want_bitpar && wf_bitpar_eligible(jobs, njobs)) {
         int Wb = 0, Kb = 64;
         if (tile && tile->strip_rows > 0) Wb = std::max(1, tile->strip_rows / 2048);
-        // Two waves (4096 rows) per strip and 64-step chunks: measured best
-        // for ED 65536^2 (3.23 ms; W = 1 3.68, W = 4 3.32; K = 128 3.39)
-        // and LCS 8192^2.
-        if (Wb == 0) Wb = max_rows > 2048 ? 2 : 1;
         if (tile && tile->tile_w > 0) Kb = tile->tile_w;
+        // The staged variant (a mask-staging wave per workgroup) when every
+        // 2048-row strip gets a CU of its own; the plain one, which packs
+        // more compute waves per CU, for larger launches.
+        bool staged;
+        if (tile && (tile->flags & WF_TILE_BITPAR_STAGED))
+            staged = true;
+        else if (tile && (tile->flags & WF_TILE_BITPAR))
+            staged = false;
+        else {
+            int64_t strips = 0;
+            for (int64_t i = 0; i < njobs; ++i)
+                if (jobs[i].n1 > 0 && jobs[i].n2 > 0) strips += (jobs[i].n2 + 2047) / 2048;
+            staged = strips <= (ctx ? ctx->prop.multiProcessorCount : 256);
+        }
+        // Strip height, measured on ED 65536^2: staged, one compute wave
+        // (2.75 ms; W = 2 3.14); plain, two waves (3.23 ms; W = 1 3.68,
+        // W = 4 3.32; K = 128 3.39).  Likewise for LCS 8192^2.
+        if (Wb == 0) Wb = staged ? 1 : (max_rows > 2048 ? 2 : 1);
+        const int Eb = staged ? WF_ENGINE_BITPAR_STAGED : WF_ENGINE_BITPAR;
         const WfVariant *best = nullptr;
         int best_d = 1 << 30;
         for (int i = 0; i < wf_num_variants_bitpar; ++i) {
             const WfVariant &v = wf_variants_bitpar[i];
+            if (v.engine != Eb) continue;
             const int d = std::abs(v.waves - Wb) * 1000 + std::abs(v.chunk - Kb);
             if (d < best_d) {
                 best_d = d;
@@ -216,7 +233,7 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
         }
         return best;
     }
-    if (tile && (tile->flags & WF_TILE_BITPAR)) return nullptr;  // asked for, not possible
+    if (tile && (tile->flags & bp_flags)) return nullptr;  // asked for, not possible
     // Default engine: a launch whose strips all fit on the chip at once is
     // bound by the wavefront's critical path, where the asynchronous engine's
     // shorter lag between waves and strips wins; a launch with many more
@@ -363,7 +380,9 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
         p.mode = mode;
         // The bit-parallel kernel indexes a 32-row table with the codes.
-        if (v->engine == WF_ENGINE_BITPAR && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;
+        if ((v->engine == WF_ENGINE_BITPAR || v->engine == WF_ENGINE_BITPAR_STAGED) &&
+            (j.problem == WF_SW || (mode & WFK_MODE_CMP)))
+            return WF_ERR_NOKERNEL;
     }
 
     // Tasks: pairs with the longest dependency chain first (columns plus the

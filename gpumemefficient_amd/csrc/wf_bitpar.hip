@@ -397,11 +397,336 @@ __global__ __launch_bounds__(64 * W) void wf_engine_bitpar(WfKernelArgs ka)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Staged variant: one more wave per workgroup looks up the match masks.
+//
+// In the engine above a compute wave spends about a sixth of its
+// instructions per step on finding Eq (the column code's LDS read, the table
+// address, the table read).  Here a staging wave does those lookups for the
+// compute waves one chunk ahead and leaves each lane's masks in LDS in step
+// order, four steps per 16-byte word, so a compute wave fetches the masks of
+// four steps with one ds_read_b128 and no address arithmetic (ED: 16.8
+// instructions per step instead of 19.6).  The staging wave also loads the
+// column codes and the strip's top boundary and stores the bottom row, so
+// the compute waves issue no global memory operation at all.  The mask
+// buffers take 16 KB of LDS per compute wave and chunk parity (W = 1: 51 KB
+// per workgroup): the planner takes this variant for launches whose strips
+// fit on the chip at once (single large pairs), and the engine above, which
+// packs more compute waves per CU, for throughput-bound batches.  Measured
+// on ED 65536^2: W = 1 2.75 ms (plain engine 3.23 ms at its best W = 2);
+// W = 2 3.14 ms, W = 3 3.93 ms (one staging wave cannot feed more).
+// ---------------------------------------------------------------------------
+
+template <int W>
+struct BsCfg {
+    static constexpr int K = 64;
+    static constexpr int D = (K + 62) / K + 1;
+    static constexpr int RS = pow2_at_least((D + 2) * K + 64);
+    static constexpr int AW = pow2_at_least((2 + (W - 1) * D) * K + 64 + K);
+};
+
+template <int W>
+struct BsShared {
+    static constexpr int K = BsCfg<W>::K;
+    static constexpr int AW = BsCfg<W>::AW;
+    static constexpr int RS = BsCfg<W>::RS;
+    // eqb[i & 1][w][q / 4][lane]: lane's match masks for steps q..q+3 of
+    // compute wave w's chunk in iteration i (written during iteration i - 1).
+    int4 eqb[2][W][K / 4][64];
+    // As in BpShared.
+    __attribute__((aligned(16))) int2 ring[W + 1][RS];
+    int peq[W][32 * 64];
+    int2 dummy[W][64 + K];
+    uint32_t aw[AW + K];
+    int task;
+    int abort_at;
+};
+
+// One chunk of K steps of a compute wave; masks from `eqw` (= eqb[i & 1][w]).
+template <int P, int K, int RS, bool MASKED>
+__device__ __forceinline__ void bs_chunk(BpState &st, int s0, int lane, int n1, const int2 *ring_prev, int2 *ring_cur,
+                                         int2 *dummy, const int4 *eqw)
+{
+    int2 *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
+    int2 *cap_last = (lane == 63) ? ring_cur + ((s0 + K) & (RS - 1)) : dummy + lane + (K - 1);
+    auto load_inj = [&](int q, int2 (&v)[4]) {
+        const int2 *ib = ring_prev + ((s0 + 64 + (q & ~63)) & (RS - 1)) + (q & 63);
+        const int4 i01 = *reinterpret_cast<const int4 *>(ib);
+        const int4 i23 = *reinterpret_cast<const int4 *>(ib + 2);
+        v[0] = make_int2(i01.x, i01.y);
+        v[1] = make_int2(i01.z, i01.w);
+        v[2] = make_int2(i23.x, i23.y);
+        v[3] = make_int2(i23.z, i23.w);
+    };
+    const int4 *ep = eqw + lane;
+    // Masks and lane 0's upper words are read one group of 4 steps ahead.
+    int4 eqv = ep[0];
+    int2 injv[4];
+    load_inj(0, injv);
+#pragma unroll
+    for (int q = 0; q < K; q += 4) {
+        int4 eqn = make_int4(0, 0, 0, 0);
+        int2 injn[4] = {};
+        if (q + 4 < K) {
+            eqn = ep[(q + 4) * 16];
+            load_inj(q + 4, injn);
+        }
+        const uint32_t e[4] = {(uint32_t)eqv.x, (uint32_t)eqv.y, (uint32_t)eqv.z, (uint32_t)eqv.w};
+        int2 capv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (MASKED) {
+                const bool valid = (unsigned)(s0 + q + u - lane) < (unsigned)n1;
+                BpState nx = st;
+                bp_step<P>(nx, injv[u], e[u]);
+                if (valid) st = nx;
+            } else {
+                bp_step<P>(st, injv[u], e[u]);
+            }
+            capv[u] = make_int2(st.ph, st.mh);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = q + u;
+            if (j == K - 1)
+                *cap_last = capv[u];
+            else
+                cap[j] = capv[u];
+        }
+        if (q + 4 < K) {
+            eqv = eqn;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) injv[u] = injn[u];
+        }
+    }
+}
+
+// One strip (`g`) of one pair: waves 0..W-1 compute, wave W stages.
+template <int P, int W>
+__device__ void bs_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g, BsShared<W> &sh, int lane,
+                         int wave, uint32_t epoch)
+{
+    constexpr int K = BsCfg<W>::K;
+    constexpr int AW = BsCfg<W>::AW;
+    constexpr int RS = BsCfg<W>::RS;
+    constexpr int kD = BsCfg<W>::D;
+    const bool stager = wave == W;
+    const int n1 = pd.n1, n2 = pd.n2;
+    const int64_t row0 = (int64_t)g * (W * kRowsPerWave) + wave * kRowsPerWave + lane * 32;
+    const uint8_t *acodes = ka.codes + pd.a_off;
+    const uint8_t *bcodes = ka.codes + pd.b_off;
+    const int64_t rows_left = (int64_t)n2 - (int64_t)g * (W * kRowsPerWave);
+    const int wa = (int)min<int64_t>(W, (rows_left + kRowsPerWave - 1) / kRowsPerWave);
+    const bool active = !stager && wave < wa;
+    const int nvalid = active ? (int)max<int64_t>(0, min<int64_t>(32, (int64_t)n2 - row0)) : 0;
+    const uint32_t rowmask = nvalid >= 32 ? 0xffffffffu : ((1u << nvalid) - 1u);
+    const bool textbook = (pd.mode & WFK_MODE_TEXTBOOK) != 0;
+
+    if (active) {
+        int *peq = sh.peq[wave];
+        for (int i = lane; i < 32 * 64; i += 64) peq[i] = 0;
+        for (int r = 0; r < nvalid; ++r) {
+            const uint32_t code = bcodes[row0 + r] & 31u;
+            atomicOr(&peq[code * 64 + lane], (int)(1u << r));
+        }
+    }
+
+    const int nsteps = n1 + 63;
+    const int nch = (nsteps + K - 1) / K;
+    const int total = nch + (wa - 1) * kD;
+    const bool has_prev = g > 0;
+    const bool has_next = g + 1 < pd.ngroups;
+    const uint64_t *bprev = ka.bnd + pd.bnd_off + (int64_t)(g - 1) * n1;
+    uint64_t *bnext = ka.bnd + pd.bnd_off + (int64_t)g * n1;
+    const uint64_t tag = (uint64_t)epoch << 32;
+
+    // Staging wave: column codes of chunk c (column c*K + lane per lane) into
+    // the code window, the strip's top boundary of chunk c into ring[0]
+    // (values as in bp_group), and the masks of an iteration's chunks.
+    auto load_code = [&](int c) -> uint32_t {
+        const int x = c * K + lane;
+        return x < n1 ? (uint32_t)acodes[x] : 0u;
+    };
+    auto put_code = [&](int c, uint32_t code) {
+        const int x = c * K + lane;
+        if (x < n1) {
+            const int slot = x & (AW - 1);
+            sh.aw[slot] = code;
+            if (slot < K) sh.aw[AW + slot] = code;
+        }
+    };
+    auto load_top = [&](int c) -> uint64_t {
+        const int x = c * K + lane;
+        return (!has_prev || x >= n1) ? tag : __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto fill_top = [&](int c, uint64_t top) -> bool {
+        const int x = c * K + lane;
+        if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)epoch) != 0) {
+            const int64_t t0 = now_ticks();
+            for (;;) {
+                __builtin_amdgcn_s_sleep(1);
+                if ((top >> 32) != (uint64_t)epoch)
+                    top = __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__builtin_amdgcn_ballot_w64((top >> 32) != (uint64_t)epoch) == 0) break;
+                if (now_ticks() - t0 > ka.timeout_ticks) return false;
+            }
+        }
+        if (x < n1) {
+            uint32_t v;
+            if (has_prev)
+                v = (uint32_t)top;
+            else
+                v = (P == WFK_ED && (x >= 1 || textbook)) ? 1u : 0u;
+            sh.ring[0][(x + 64) & (RS - 1)] = (P == WFK_ED)
+                                                  ? make_int2((int)((v & 1u) << 31), (int)(((v >> 1) & 1u) << 31))
+                                                  : make_int2((int)((v & 1u) << 31), 0);
+        }
+        return true;
+    };
+    // Masks of iteration `it`: compute wave w runs chunk it - w*D; its lane
+    // l at step q reads column (it - w*D)*K + q - l (codes of columns
+    // outside the pair are stale, masked to a table row; those steps keep
+    // their state).
+    auto produce = [&](int it) {
+        for (int w = 0; w < wa; ++w) {
+            const int c = it - w * kD;
+            if (c < 0 || c >= nch) continue;
+            const uint32_t *cp = sh.aw + ((c * K - lane) & (AW - 1));
+            const int *pq = sh.peq[w] + lane;
+            int4 *d = &sh.eqb[it & 1][w][0][lane];
+            // All codes, then all table reads, then the stores: the table
+            // reads issue back to back instead of each group waiting out two
+            // dependent LDS round trips.
+            uint32_t code[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) code[q] = cp[q];
+            int e[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) e[q] = pq[(code[q] & 31u) * 64];
+#pragma unroll
+            for (int gq = 0; gq < K / 4; ++gq) d[gq * 64] = make_int4(e[4 * gq], e[4 * gq + 1], e[4 * gq + 2], e[4 * gq + 3]);
+        }
+    };
+
+    __syncthreads();  // match-mask tables built
+    uint32_t code_next = 0u;  // stager: codes of chunk j + 2, loaded in iteration j
+    if (stager) {
+        put_code(0, load_code(0));
+        if (1 < nch) code_next = load_code(1);
+        produce(0);
+        if (!fill_top(0, load_top(0)) && lane == 0) sh.abort_at = -1;
+    }
+    __syncthreads();
+
+    int copied = 0;  // stager: columns already stored to bnext
+    auto copy_out = [&](int upto, bool from_ring) {
+        while (copied < upto) {
+            const int x = copied + lane;
+            if (x < upto) {
+                uint32_t v = 0;
+                if (from_ring) {
+                    const int2 w = sh.ring[W][(x + 64) & (RS - 1)];
+                    v = (P == WFK_ED) ? (((uint32_t)w.x >> 31) | (((uint32_t)w.y >> 31) << 1)) : ((uint32_t)w.x >> 31);
+                }
+                __hip_atomic_store(bnext + x, tag | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            copied = min(upto, copied + 64);
+        }
+    };
+
+    BpState st;
+    st.pv = 0xffffffffu;  // ED: column 0 rises by 1 per row; LCS: V = all ones
+    st.mv = 0u;
+    st.ph = 0;
+    st.mh = 0;
+
+    bool aborted = sh.abort_at <= -1;
+    for (int j = 0; j < total && !aborted; ++j) {
+        if (stager) {
+            // The bottom row's columns the last compute wave finished in
+            // iteration j - 1, to HBM for the strip below (here rather than
+            // in that wave, whose next chunk would otherwise wait for the
+            // stores to complete).
+            const int cl = j - 1 - (wa - 1) * kD;
+            if (has_next && cl >= 0 && cl < nch) copy_out(min(n1, cl * K + K - 63), true);
+            if (j + 1 < nch) put_code(j + 1, code_next);
+            if (j + 2 < nch) code_next = load_code(j + 2);
+            const uint64_t top = (j + 1 < nch) ? load_top(j + 1) : tag;
+            if (j + 1 < total) produce(j + 1);
+            if (j + 1 < nch && !fill_top(j + 1, top) && lane == 0) sh.abort_at = j;
+        } else {
+            const int cj = j - wave * kD;
+            if (cj >= 0 && cj < nch && active) {
+                const int s0 = cj * K;
+                const bool steady = (s0 >= 63) && (s0 + K - 1 <= n1 - 1);
+                const int4 *eqw = &sh.eqb[j & 1][wave][0][0];
+                if (steady)
+                    bs_chunk<P, K, RS, false>(st, s0, lane, n1, sh.ring[wave], sh.ring[wave + 1], sh.dummy[wave], eqw);
+                else
+                    bs_chunk<P, K, RS, true>(st, s0, lane, n1, sh.ring[wave], sh.ring[wave + 1], sh.dummy[wave], eqw);
+            }
+        }
+        __syncthreads();
+        aborted = sh.abort_at <= j;
+    }
+
+    if (aborted) {
+        if (threadIdx.x == 0) atomicOr(&ka.res[pair].status, 1);
+        if (stager && has_next) copy_out(n1, false);
+        return;
+    }
+    if (stager && has_next) copy_out(n1, true);
+
+    if (active) {
+        int v;
+        if (P == WFK_ED)
+            v = __builtin_popcount(st.pv & rowmask) - __builtin_popcount(st.mv & rowmask);
+        else
+            v = __builtin_popcount(~st.pv & rowmask);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (P == WFK_ED && g == 0 && wave == 0) v += textbook ? n1 : n1 - 1;  // D(0, n1)
+        if (lane == 0) atomicAdd(&ka.res[pair].last, v);
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(64 * (W + 1)) void wf_engine_bitpar_staged(WfKernelArgs ka)
+{
+    __shared__ BsShared<W> sh;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t epoch = __builtin_amdgcn_readfirstlane(*ka.epoch);
+    for (int i = threadIdx.x; i < BsShared<W>::AW + BsShared<W>::K; i += blockDim.x) sh.aw[i] = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sh.task = ka.task_begin + atomicAdd(ka.counter, 1);
+            sh.abort_at = 0x7fffffff;
+        }
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(sh.task);
+        __syncthreads();
+        if (t >= ka.task_end) break;
+        const WfTaskDev td = ka.tasks[t];
+        const WfPairDev pd = ka.pairs[td.pair];
+        wfk::set_wave_priority(pd.prio);
+        if (pd.prob == WFK_ED)
+            bs_group<WFK_ED, W>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
+        else
+            bs_group<WFK_LCS, W>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
+        __syncthreads();
+    }
+}
+
 }  // namespace wfb
+
+#define WF_BITPAR_STAGED(W_) \
+    {WF_ENGINE_BITPAR_STAGED, 32, W_, 64, 64 * ((W_) + 1), reinterpret_cast<const void *>(&wfb::wf_engine_bitpar_staged<W_>)}
 
 #define WF_BITPAR(W_, K_) \
     {WF_ENGINE_BITPAR, 32, W_, K_, 64 * (W_), reinterpret_cast<const void *>(&wfb::wf_engine_bitpar<W_, K_>)}
 
-extern "C" const WfVariant wf_variants_bitpar[] = {WF_BITPAR(1, 64), WF_BITPAR(2, 64), WF_BITPAR(4, 64),
-                                                   WF_BITPAR(1, 128), WF_BITPAR(2, 128), WF_BITPAR(4, 128)};
+extern "C" const WfVariant wf_variants_bitpar[] = {WF_BITPAR(1, 64),        WF_BITPAR(2, 64),        WF_BITPAR(4, 64),
+                                                   WF_BITPAR(1, 128),       WF_BITPAR(2, 128),       WF_BITPAR(4, 128),
+                                                   WF_BITPAR_STAGED(1),     WF_BITPAR_STAGED(2)};
 extern "C" const int wf_num_variants_bitpar = sizeof(wf_variants_bitpar) / sizeof(wf_variants_bitpar[0]);

@@ -130,7 +130,7 @@ int wf_tune_save(const char *path)
     FILE *f = std::fopen(path, "w");
     if (!f) return WF_ERR_INVALID;
     std::fprintf(f, "# wfdp autotune table: arch cus problem(0 LCS,1 ED,2 SW) class bucket "
-                    "engine(0 lockstep,1 async,2 bitpar) rows_per_lane waves chunk ms\n");
+                    "engine(0 lockstep,1 async,2 bitpar,3 bitpar staged) rows_per_lane waves chunk ms\n");
     std::lock_guard<std::mutex> lk(g_mu);
     for (const TuneEntry &e : g_table)
         std::fprintf(f, "%s %d %d %s %d %d %d %d %d %.4f\n", e.arch.c_str(), e.cus, e.problem,
@@ -148,7 +148,10 @@ int wf_tune_clear(void)
 
 static int32_t engine_flag(int engine)
 {
-    return engine == WF_ENGINE_LOCKSTEP ? WF_TILE_LOCKSTEP : engine == WF_ENGINE_ASYNC ? WF_TILE_ASYNC : WF_TILE_BITPAR;
+    return engine == WF_ENGINE_LOCKSTEP ? WF_TILE_LOCKSTEP
+           : engine == WF_ENGINE_ASYNC  ? WF_TILE_ASYNC
+           : engine == WF_ENGINE_BITPAR ? WF_TILE_BITPAR
+                                        : WF_TILE_BITPAR_STAGED;
 }
 
 int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_tile *best, double *best_ms)

@@ -9,7 +9,7 @@
 
 // WF_ENGINE_BITPAR: LCS and ED only, 32 rows per lane as one bit-vector word
 // (wf_bitpar.hip).
-enum { WF_ENGINE_LOCKSTEP = 0, WF_ENGINE_ASYNC = 1, WF_ENGINE_BITPAR = 2 };
+enum { WF_ENGINE_LOCKSTEP = 0, WF_ENGINE_ASYNC = 1, WF_ENGINE_BITPAR = 2, WF_ENGINE_BITPAR_STAGED = 3 };
 
 struct WfVariant {
     int engine;         // WF_ENGINE_*

@@ -80,6 +80,10 @@ typedef struct {
 #define WF_TILE_BITPAR 0x20     /* bit-parallel engine: LCS and ED with at most 32 symbols,
                                    32 rows per lane as one bit-vector word; the default for
                                    such pairs unless a cell engine flag is given */
+#define WF_TILE_BITPAR_STAGED 0x40 /* bit-parallel engine with a mask-staging wave per
+                                      workgroup (one workgroup per CU); without either
+                                      bit-parallel flag the planner takes it when every
+                                      strip of the launch gets a CU of its own */
 #define WF_TILE_NO_COPYBACK 0x10 /* wf_stencil: leave the host table unchanged (the
                                     reference never copies its result back) */
 

@@ -44,7 +44,7 @@ def test_non_square(ctx, prob):
 @pytest.mark.parametrize("v", VARIANTS, ids=lambda v: f"{v['engine']}-R{v['rows_per_lane']}W{v['waves']}K{v['chunk']}")
 @pytest.mark.parametrize("prob", ["LCS", "ED", "SW"])
 def test_every_variant(ctx, prob, v):
-    if v["engine"] == "bitpar" and prob == "SW":
+    if v["engine"].startswith("bitpar") and prob == "SW":
         pytest.skip("the bit-parallel engine takes LCS and ED only (SW falls back to a cell engine)")
     R, W, K = v["rows_per_lane"], v["waves"], v["chunk"]
     rng = np.random.default_rng(100 + R * 31 + W * 7 + K)
@@ -79,10 +79,12 @@ def test_arbitrary_symbols(ctx, prob):
 
 
 BITPAR = Tile(engine="bitpar")
+BITPAR_STAGED = Tile(engine="bitpar_staged")
 
 
+@pytest.mark.parametrize("tile", [BITPAR, BITPAR_STAGED], ids=["plain", "staged"])
 @pytest.mark.parametrize("prob,textbook", [("LCS", False), ("ED", False), ("ED", True)])
-def test_bitpar_word_and_wave_edges(ctx, prob, textbook):
+def test_bitpar_word_and_wave_edges(ctx, prob, textbook, tile):
     """The bit-parallel engine at the edges of its geometry: rows around one
     32-row word, one 64-lane wave (2048 rows) and a 4-wave strip, columns
     around one wave's 64-step skew, 2- to 32-letter alphabets."""
@@ -90,14 +92,19 @@ def test_bitpar_word_and_wave_edges(ctx, prob, textbook):
     for n1, n2, alpha in [(1, 1, 4), (2, 31, 2), (3, 32, 4), (64, 33, 4), (63, 2047, 20), (65, 2048, 32),
                           (200, 2049, 4), (1, 8193, 4), (2500, 8192 * 2 + 3, 20), (4097, 100, 3)]:
         a, b = rand_pair(rng, n1, n2, alpha)
-        check(ctx, prob, a, b, tile=BITPAR, textbook=textbook)
+        check(ctx, prob, a, b, tile=tile, textbook=textbook)
 
 
 def test_bitpar_is_the_default_for_lcs_and_ed(ctx):
     rng = np.random.default_rng(32)
     a, b = rand_pair(rng, 3000, 5000, 20)
     for prob in ("LCS", "ED"):
+        # One pair of two strips: the chip is underfilled, so the staged variant.
         plan = ctx.plan([Job(prob, a, b)])
+        assert plan.describe()["engine"] == "bitpar_staged"
+        plan.close()
+        # Hundreds of pairs: more strips than CUs, the plain variant.
+        plan = ctx.plan([Job(prob, a[:500], b[:300])] * 600)
         assert plan.describe()["engine"] == "bitpar"
         plan.close()
     for prob, alpha in (("SW", 4), ("ED", 40)):  # SW, and more than 32 symbols: cell engines
