@@ -323,14 +323,18 @@ int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, i
 // Planning
 // ----------------------------------------------------------------------------
 
+// Column split of bit-parallel pairs (WfSplitDev): on unless WFDP_SPLIT=0.
+static bool split_enabled()
+{
+    const char *e = std::getenv("WFDP_SPLIT");
+    return !(e && e[0] == '0');
+}
+// Pairs narrower than this are not split.
+constexpr int64_t kSplitMinCols = 1024;
+
 int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                   WfPlanHost *ph)
 {
-    ph->npairs = njobs;
-    ph->pairs.assign(njobs, WfPairDev{});
-    ph->host_result.assign(njobs, 0);
-    ph->host_only.assign(njobs, 0);
-    ph->cells.assign(njobs, 0);
     int64_t max_rows = 0, total_cells = 0;
     for (int64_t i = 0; i < njobs; ++i) {
         const wf_job &j = jobs[i];
@@ -339,34 +343,95 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if ((j.n1 > 0 && !j.a) || (j.n2 > 0 && !j.b)) return WF_ERR_INVALID;
         if (j.n1 > 0 && j.n2 > 0) max_rows = std::max(max_rows, j.n2);
         total_cells += j.n1 * j.n2;
-        ph->cells[i] = (uint64_t)(j.n1 * j.n2);
     }
     const WfVariant *v = wf_pick_variant(ctx, tile, jobs, njobs, max_rows, total_cells);
     if (!v) return WF_ERR_NOKERNEL;
     ph->variant = v;
     const int64_t strip = 64LL * v->waves * v->rows_per_lane;
+    const bool bitpar = v->engine == WF_ENGINE_BITPAR || v->engine == WF_ENGINE_BITPAR_STAGED;
 
-    // Codes: a then b per pair, 16-byte aligned.
+    // Bit-parallel pairs are split into two column halves computed at once
+    // (WfSplitDev) when the launch has room for twice its strips: a single
+    // pair's time is its column count times the step time, so the halves
+    // finish in about half of it.
+    std::vector<int64_t> mid(njobs, 0);
+    int64_t nsplit = 0;
+    if (bitpar && split_enabled()) {
+        int64_t strips = 0;
+        for (int64_t i = 0; i < njobs; ++i)
+            if (jobs[i].n1 > 0 && jobs[i].n2 > 0) strips += (jobs[i].n2 + strip - 1) / strip;
+        const int64_t cus = ctx ? ctx->prop.multiProcessorCount : 256;
+        if (2 * strips <= cus)
+            for (int64_t i = 0; i < njobs; ++i)
+                if (jobs[i].n1 >= kSplitMinCols && jobs[i].n2 > 0) {
+                    mid[i] = jobs[i].n1 / 2;
+                    ++nsplit;
+                }
+    }
+    const int64_t np = njobs + nsplit;
+    ph->njobs = njobs;
+    ph->npairs = np;
+    ph->pairs.assign(np, WfPairDev{});
+    ph->host_result.assign(np, 0);
+    ph->host_only.assign(np, 0);
+    ph->cells.assign(np, 0);
+    ph->splits.clear();
+    ph->colw_entries = 0;
+    for (int64_t i = 0; i < njobs; ++i) ph->cells[i] = (uint64_t)(jobs[i].n1 * jobs[i].n2);
+
+    // Codes: a then b per pair, 16-byte aligned; a split pair's reverse half
+    // gets its own reversed copies after all jobs.
     int64_t code_bytes = 0, bnd_ints = 0;
+    auto place = [&](WfPairDev &p, int prob, int64_t n1, int64_t n2, int64_t a_bytes) {
+        p.prob = prob;
+        p.n1 = (int32_t)n1;
+        p.n2 = (int32_t)n2;
+        p.a_off = code_bytes;
+        code_bytes += (a_bytes + 15) & ~15LL;
+        p.b_off = code_bytes;
+        code_bytes += (n2 + 15) & ~15LL;
+        p.ngroups = (int32_t)((n2 + strip - 1) / strip);
+        p.bnd_off = bnd_ints;
+        bnd_ints += (int64_t)(p.ngroups - 1) * n1;
+    };
     for (int64_t i = 0; i < njobs; ++i) {
         const wf_job &j = jobs[i];
         WfPairDev &p = ph->pairs[i];
-        p.prob = j.problem;
-        p.n1 = (int32_t)j.n1;
-        p.n2 = (int32_t)j.n2;
-        p.a_off = code_bytes;
-        code_bytes += (j.n1 + 15) & ~15LL;
-        p.b_off = code_bytes;
-        code_bytes += (j.n2 + 15) & ~15LL;
         if (j.n1 == 0 || j.n2 == 0) {
+            p.prob = j.problem;
+            p.n1 = (int32_t)j.n1;
+            p.n2 = (int32_t)j.n2;
+            p.a_off = code_bytes;
+            code_bytes += (j.n1 + 15) & ~15LL;
+            p.b_off = code_bytes;
+            code_bytes += (j.n2 + 15) & ~15LL;
             ph->host_only[i] = 1;
             ph->host_result[i] = empty_score(j.problem, j.n1, j.n2, j.flags);
             p.ngroups = 0;
             continue;
         }
-        p.ngroups = (int32_t)((j.n2 + strip - 1) / strip);
-        p.bnd_off = bnd_ints;
-        bnd_ints += (int64_t)(p.ngroups - 1) * j.n1;
+        place(p, j.problem, mid[i] ? mid[i] : j.n1, j.n2, j.n1);
+    }
+    {
+        int64_t r = njobs;
+        for (int64_t i = 0; i < njobs; ++i) {
+            if (!mid[i]) continue;
+            const wf_job &j = jobs[i];
+            place(ph->pairs[r], j.problem, j.n1 - mid[i], j.n2, j.n1 - mid[i]);
+            WfSplitDev sp{};
+            sp.job = (int32_t)i;
+            sp.rev = (int32_t)r;
+            sp.prob = j.problem;
+            sp.n2 = (int32_t)j.n2;
+            ph->pairs[i].col_off = ph->colw_entries;
+            sp.fwd_off = ph->colw_entries;
+            ph->colw_entries += (int64_t)ph->pairs[i].ngroups * v->waves * 64;
+            ph->pairs[r].col_off = ph->colw_entries;
+            sp.rev_off = ph->colw_entries;
+            ph->colw_entries += (int64_t)ph->pairs[r].ngroups * v->waves * 64;
+            ph->splits.push_back(sp);
+            ++r;
+        }
     }
     ph->codes.assign((size_t)std::max<int64_t>(code_bytes, 16), 0);
     for (int64_t i = 0; i < njobs; ++i) {
@@ -380,9 +445,26 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
         p.mode = mode;
         // The bit-parallel kernel indexes a 32-row table with the codes.
-        if ((v->engine == WF_ENGINE_BITPAR || v->engine == WF_ENGINE_BITPAR_STAGED) &&
-            (j.problem == WF_SW || (mode & WFK_MODE_CMP)))
-            return WF_ERR_NOKERNEL;
+        if (bitpar && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;
+    }
+    for (WfSplitDev &sp : ph->splits) {
+        // Reverse half: a[mid..n1) and b reversed, textbook ED boundary (its
+        // row 0 is a plain distance from the empty prefix).
+        const wf_job &j = jobs[sp.job];
+        const WfPairDev &f = ph->pairs[sp.job];
+        WfPairDev &r = ph->pairs[sp.rev];
+        const int64_t m = mid[sp.job], nr = j.n1 - m;
+        const uint8_t *fa = ph->codes.data() + f.a_off;
+        const uint8_t *fb = ph->codes.data() + f.b_off;
+        uint8_t *ra = ph->codes.data() + r.a_off;
+        uint8_t *rb = ph->codes.data() + r.b_off;
+        for (int64_t k = 0; k < nr; ++k) ra[k] = fa[j.n1 - 1 - k];
+        for (int64_t k = 0; k < j.n2; ++k) rb[k] = fb[j.n2 - 1 - k];
+        r.mode = (f.mode & ~WFK_MODE_TEXTBOOK) | (j.problem == WF_ED ? WFK_MODE_TEXTBOOK : 0);
+        if (j.problem == WF_ED) {
+            sp.f0 = (int32_t)((f.mode & WFK_MODE_TEXTBOOK) ? m : m - 1);  // row 0 at column mid
+            sp.r0 = (int32_t)nr;
+        }
     }
 
     // Tasks: pairs with the longest dependency chain first (columns plus the
@@ -391,10 +473,10 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     // deadlock-free.  The longest chains also get the highest wave issue
     // priority, so that co-resident strips of short pairs fill the issue
     // slots they leave idle instead of stretching the critical path.
-    std::vector<int64_t> chain(njobs, 0);
+    std::vector<int64_t> chain(np, 0);
     int64_t max_chain = 1;
     const int64_t strip_lag = (int64_t)v->waves * (64 + 2 * v->chunk);
-    for (int64_t i = 0; i < njobs; ++i) {
+    for (int64_t i = 0; i < np; ++i) {
         const WfPairDev &p = ph->pairs[i];
         if (p.ngroups == 0) continue;
         chain[i] = p.n1 + 63 + (int64_t)p.ngroups * strip_lag;
@@ -404,18 +486,18 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         const char *e = std::getenv("WFDP_PRIORITY");
         return !(e && e[0] == '0');
     }();
-    for (int64_t i = 0; i < njobs; ++i)
+    for (int64_t i = 0; i < np; ++i)
         ph->pairs[i].prio = use_prio ? (int32_t)std::min<int64_t>(3, 4 * chain[i] / (max_chain + 1)) : 0;
-    std::vector<int64_t> order(njobs);
+    std::vector<int64_t> order(np);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
         if (chain[x] != chain[y]) return chain[x] > chain[y];
         return ph->cells[x] > ph->cells[y];
     });
     ph->tasks.clear();
-    ph->task_begin.assign(njobs, 0);
-    ph->task_end.assign(njobs, 0);
-    for (int64_t k = 0; k < njobs; ++k) {
+    ph->task_begin.assign(np, 0);
+    ph->task_end.assign(np, 0);
+    for (int64_t k = 0; k < np; ++k) {
         const int64_t i = order[k];
         ph->task_begin[i] = (int64_t)ph->tasks.size();
         for (int32_t g = 0; g < ph->pairs[i].ngroups; ++g) ph->tasks.push_back(WfTaskDev{(int32_t)i, g});
@@ -444,8 +526,15 @@ int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t 
     if ((st = ar->counters.ensure((size_t)(ph->npairs + 1) * sizeof(int32_t))) != WF_OK) return st;
     // Launch epoch: zero on allocation, so it never matches a zeroed boundary word.
     if ((st = ar->epoch.ensure(sizeof(uint32_t), true)) != WF_OK) return st;
+    if ((st = ar->colw.ensure((size_t)std::max<int64_t>(1, ph->colw_entries) * 2 * sizeof(uint32_t))) != WF_OK)
+        return st;
+    if ((st = ar->splits.ensure(std::max<size_t>(1, ph->splits.size()) * sizeof(WfSplitDev))) != WF_OK) return st;
     if (hipMemcpyAsync(ar->codes.ptr, ph->codes.data(), ph->codes.size(), hipMemcpyHostToDevice,
                        stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    if (!ph->splits.empty() &&
+        hipMemcpyAsync(ar->splits.ptr, ph->splits.data(), ph->splits.size() * sizeof(WfSplitDev),
+                       hipMemcpyHostToDevice, stream) != hipSuccess)
         return WF_ERR_DEVICE;
     if (!ph->pairs.empty() &&
         hipMemcpyAsync(ar->pairs.ptr, ph->pairs.data(), ph->pairs.size() * sizeof(WfPairDev),
@@ -474,27 +563,10 @@ int wf_max_resident_groups(wf_ctx *ctx, const WfVariant *v)
     return per_cu * ctx->prop.multiProcessorCount;
 }
 
-// Enqueue the reset kernel and the engine launch(es) of a plan on the
-// context stream (also used inside hipGraph capture: no memset nodes, and
-// the epoch that tags boundary values is advanced on the device).
-int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int maxg, hipStream_t stream)
+// The engine launch(es) of a plan: one persistent launch, or one per pair.
+static int enqueue_engine(WfPlanHost *ph, const WfVariant *v, int lim, int mode, WfKernelArgs ka, WfArenas *ar,
+                          hipStream_t stream)
 {
-    const int64_t ncounters = ph->npairs + 1;
-    if (wf_launch_reset((WfResultDev *)ar->res.ptr, ph->npairs, (int32_t *)ar->counters.ptr, ncounters,
-                        (uint32_t *)ar->epoch.ptr, stream) != hipSuccess)
-        return WF_ERR_DEVICE;
-    WfKernelArgs ka{};
-    ka.pairs = (const WfPairDev *)ar->pairs.ptr;
-    ka.tasks = (const WfTaskDev *)ar->tasks.ptr;
-    ka.codes = (const uint8_t *)ar->codes.ptr;
-    ka.bnd = (uint64_t *)ar->bnd.ptr;
-    ka.res = (WfResultDev *)ar->res.ptr;
-    ka.epoch = (const uint32_t *)ar->epoch.ptr;
-    ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
-    ph->grid = 0;
-    if (ph->tasks.empty()) return WF_OK;
-    const WfVariant *v = ph->variant;
-    const int lim = maxg > 0 ? maxg : wf_max_resident_groups(ctx, v);
     if (mode == WF_RUN_PER_PAIR) {
         for (int64_t k = 0; k < ph->npairs; ++k) {
             const int64_t i = ph->order[k];
@@ -514,6 +586,37 @@ int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int max
     ka.task_end = (int32_t)ph->tasks.size();
     ph->grid = (int)std::min<int64_t>((int64_t)ph->tasks.size(), lim);
     if (wf_launch_engine(v, ph->grid, &ka, stream) != hipSuccess) return WF_ERR_DEVICE;
+    return WF_OK;
+}
+
+// Enqueue the reset kernel and the engine launch(es) of a plan on the
+// context stream (also used inside hipGraph capture: no memset nodes, and
+// the epoch that tags boundary values is advanced on the device).
+int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int maxg, hipStream_t stream)
+{
+    const int64_t ncounters = ph->npairs + 1;
+    if (wf_launch_reset((WfResultDev *)ar->res.ptr, ph->npairs, (int32_t *)ar->counters.ptr, ncounters,
+                        (uint32_t *)ar->epoch.ptr, stream) != hipSuccess)
+        return WF_ERR_DEVICE;
+    WfKernelArgs ka{};
+    ka.pairs = (const WfPairDev *)ar->pairs.ptr;
+    ka.tasks = (const WfTaskDev *)ar->tasks.ptr;
+    ka.codes = (const uint8_t *)ar->codes.ptr;
+    ka.bnd = (uint64_t *)ar->bnd.ptr;
+    ka.res = (WfResultDev *)ar->res.ptr;
+    ka.epoch = (const uint32_t *)ar->epoch.ptr;
+    ka.timeout_ticks = (int64_t)(ctx->timeout_s * 1.0e8);
+    ka.colw = (uint32_t *)ar->colw.ptr;
+    ph->grid = 0;
+    if (ph->tasks.empty()) return WF_OK;
+    const WfVariant *v = ph->variant;
+    const int lim = maxg > 0 ? maxg : wf_max_resident_groups(ctx, v);
+    const int st = enqueue_engine(ph, v, lim, mode, ka, ar, stream);
+    if (st != WF_OK) return st;
+    if (!ph->splits.empty() &&
+        wf_launch_bitpar_combine((const WfSplitDev *)ar->splits.ptr, (int)ph->splits.size(),
+                                 (const uint32_t *)ar->colw.ptr, (WfResultDev *)ar->res.ptr, stream) != hipSuccess)
+        return WF_ERR_DEVICE;
     return WF_OK;
 }
 
@@ -548,7 +651,7 @@ int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel
         return WF_ERR_DEVICE;
     (void)ctx;
     int worst = WF_OK;
-    for (int64_t i = 0; i < std::min(n, ph->npairs); ++i) {
+    for (int64_t i = 0; i < std::min(n, ph->njobs); ++i) {
         wf_result &o = out[i];
         std::memset(&o, 0, sizeof(o));
         o.cells = ph->cells[i];

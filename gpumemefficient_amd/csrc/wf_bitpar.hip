@@ -365,6 +365,11 @@ __device__ void bp_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, 
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (P == WFK_ED && g == 0 && wave == 0) v += textbook ? n1 : n1 - 1;  // D(0, n1)
         if (lane == 0) atomicAdd(&ka.res[pair].last, v);
+        if (pd.col_off >= 0) {
+            uint32_t *cw = ka.colw + 2 * (pd.col_off + ((int64_t)g * W + wave) * 64 + lane);
+            cw[0] = st.pv;
+            cw[1] = st.mv;
+        }
     }
 }
 
@@ -687,6 +692,106 @@ __device__ void bs_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, 
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (P == WFK_ED && g == 0 && wave == 0) v += textbook ? n1 : n1 - 1;  // D(0, n1)
         if (lane == 0) atomicAdd(&ka.res[pair].last, v);
+        if (pd.col_off >= 0) {
+            uint32_t *cw = ka.colw + 2 * (pd.col_off + ((int64_t)g * W + wave) * 64 + lane);
+            cw[0] = st.pv;
+            cw[1] = st.mv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split pairs: the score from the two halves' last-column words.
+//
+// Row r (0-based, r < n2) of the forward half's column mid changes by
+// d_r = Pv_r - Mv_r (ED) or z_r = 1 - V_r (LCS, the LCS grows by one where V
+// has a zero); F(y) = f0 + sum_{r<y} d_r.  The reverse half's last column
+// at reverse row n2 - y is R(y) = r0 + T' - sum_{r<y} d'_{n2-1-r}, T' the sum
+// of all its deltas.  So F(y) + R(y) = f0 + r0 + T' + sum_{r<y} e_r with
+// e_r = d_r - d'_{n2-1-r}, and the score is that constant plus the smallest
+// (ED) or largest (LCS) prefix sum of e, the empty prefix included.  One
+// workgroup per split; the reverse words are read bit-reversed so that both
+// halves line up row by row.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void wf_bitpar_combine(const WfSplitDev *splits, const uint32_t *colw,
+                                                          WfResultDev *res)
+{
+    const WfSplitDev sp = splits[blockIdx.x];
+    const bool ed = sp.prob == WFK_ED;
+    const int n2 = sp.n2;
+    const int nw = (n2 + 31) >> 5;
+    const uint32_t *fw = colw + 2 * sp.fwd_off;
+    const uint32_t *rw = colw + 2 * sp.rev_off;
+    // Bits of reverse rows [base, base + 32) (base > -32), bit k = row
+    // base + k; rows below 0 read as 0.  h: 0 Pv/V, 1 Mv.
+    auto rev_bits = [&](int base, int h) -> uint32_t {
+        const int lo = base >= 0 ? (base >> 5) : -1;
+        const int sh = base - 32 * lo;
+        const uint32_t wl = lo >= 0 ? rw[2 * lo + h] : 0u;
+        const uint32_t wh = (lo + 1 < nw) ? rw[2 * (lo + 1) + h] : 0u;
+        return (uint32_t)((((uint64_t)wh << 32) | wl) >> sh);
+    };
+    const int tid = threadIdx.x;
+    const int per = (nw + 1023) >> 10;
+    const int w0 = min(nw, tid * per), w1 = min(nw, w0 + per);
+    int run = 0, best = 0, rtot = 0;
+    for (int w = w0; w < w1; ++w) {
+        const int nk = min(32, n2 - 32 * w);
+        const uint32_t m = nk >= 32 ? 0xffffffffu : ((1u << nk) - 1u);
+        const int base = n2 - 32 * w - 32;  // reverse rows of original rows 32w+31 .. 32w
+        uint32_t fa, fb, ra, rb;
+        if (ed) {
+            fa = fw[2 * w] & m;
+            fb = fw[2 * w + 1] & m;
+            ra = __builtin_bitreverse32(rev_bits(base, 0)) & m;
+            rb = __builtin_bitreverse32(rev_bits(base, 1)) & m;
+            rtot += __builtin_popcount(ra) - __builtin_popcount(rb);
+        } else {
+            fa = ~fw[2 * w] & m;  // zeros of V: the LCS grows at that row
+            fb = 0u;
+            ra = __builtin_bitreverse32(~rev_bits(base, 0)) & m;
+            rb = 0u;
+            rtot += __builtin_popcount(ra);
+        }
+        for (int k = 0; k < nk; ++k) {
+            run += (int)((fa >> k) & 1u) - (int)((fb >> k) & 1u) - (int)((ra >> k) & 1u) + (int)((rb >> k) & 1u);
+            best = ed ? min(best, run) : max(best, run);
+        }
+    }
+    // Exclusive scan of the threads' sums (in word order) and the reductions.
+    __shared__ int wsum[16], wbest[16], wrt[16];
+    const int lane = tid & 63, wv = tid >> 6;
+    int incl = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    int rt = rtot;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rt += __shfl_xor(rt, off);
+    if (lane == 0) wrt[wv] = rt;
+    __syncthreads();
+    int before = 0, rall = 0;
+    for (int i = 0; i < 16; ++i) {
+        if (i < wv) before += wsum[i];
+        rall += wrt[i];
+    }
+    int cand = before + incl - run + best;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(cand, off);
+        cand = ed ? min(cand, o) : max(cand, o);
+    }
+    if (lane == 0) wbest[wv] = cand;
+    __syncthreads();
+    if (tid == 0) {
+        int b = wbest[0];
+        for (int i = 1; i < 16; ++i) b = ed ? min(b, wbest[i]) : max(b, wbest[i]);
+        res[sp.job].last = sp.f0 + sp.r0 + rall + b;
+        res[sp.job].status |= res[sp.rev].status;
     }
 }
 
@@ -719,6 +824,14 @@ __global__ __launch_bounds__(64 * (W + 1)) void wf_engine_bitpar_staged(WfKernel
 }
 
 }  // namespace wfb
+
+extern "C" hipError_t wf_launch_bitpar_combine(const WfSplitDev *splits, int nsplits, const uint32_t *colw,
+                                               WfResultDev *res, hipStream_t stream)
+{
+    if (nsplits <= 0) return hipSuccess;
+    hipLaunchKernelGGL(wfb::wf_bitpar_combine, dim3(nsplits), dim3(1024), 0, stream, splits, colw, res);
+    return hipGetLastError();
+}
 
 #define WF_BITPAR_STAGED(W_) \
     {WF_ENGINE_BITPAR_STAGED, 32, W_, 64, 64 * ((W_) + 1), reinterpret_cast<const void *>(&wfb::wf_engine_bitpar_staged<W_>)}

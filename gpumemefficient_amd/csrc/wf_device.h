@@ -30,6 +30,28 @@ struct WfPairDev {
     int32_t mode;      // WFK_MODE_* bits
     int32_t prio;      // wave issue priority (s_setprio 0..3) of this pair's strips
     int32_t ngroups;   // strips of (waves * 64 * rows_per_lane) rows
+    // Bit-parallel engine: entry offset in WfKernelArgs::colw where each
+    // 32-row word stores its last column's vertical deltas (two words per
+    // entry, ED: Pv, Mv; LCS: V, 0), at entry (strip * waves + wave) * 64 +
+    // lane; -1: not stored.
+    int64_t col_off = -1;
+};
+
+// A pair computed as two column halves (bit-parallel engine, LCS and ED):
+// pair `job` sweeps columns 1..mid forward, pair `rev` sweeps the reversed
+// columns mid+1..n1 against the reversed rows.  With F(y) the forward
+// half's column mid and R(y) the reverse half's last column at row n2 - y,
+// the score is min (ED) / max (LCS) over y of F(y) + R(y) (Hirschberg's
+// split, J. ACM 1975; CACM 18(6) 1975 for LCS).
+struct WfSplitDev {
+    int32_t job;       // result slot (the forward half's pair)
+    int32_t rev;       // the reverse half's pair
+    int32_t prob;      // WFK_ED / WFK_LCS
+    int32_t n2;        // rows
+    int32_t f0;        // ED: F(0), row 0 of the table at column mid; LCS: 0
+    int32_t r0;        // ED: the reverse half's row 0 at its last column (n1 - mid); LCS: 0
+    int64_t fwd_off;   // colw entry offsets of the two halves
+    int64_t rev_off;
 };
 
 // One unit of persistent work: strip `group` of pair `pair`.  Tasks of a
@@ -61,6 +83,7 @@ struct WfKernelArgs {
     const uint32_t *epoch;   // launch number (device memory, bumped by the reset kernel before
                              // every launch, so graph replays get fresh tags)
     int32_t *table;          // table-dump kernel only: H(y+1, x+1) of pair 0 at [y * n1 + x]
+    uint32_t *colw;          // bit-parallel engine: last-column entries, two words each (WfPairDev::col_off)
 };
 
 // In-place table stencils swept with the same strip schedule (wf_stencil.hip):

@@ -50,9 +50,11 @@ struct DevBuf {
 };
 
 struct WfArenas {
-    DevBuf codes, pairs, tasks, bnd, res, counters, epoch;
+    DevBuf codes, pairs, tasks, bnd, res, counters, epoch, splits, colw;
     void release()
     {
+        splits.release();
+        colw.release();
         epoch.release();
         codes.release();
         pairs.release();
@@ -73,7 +75,10 @@ struct wf_ctx {
 };
 
 struct WfPlanHost {
-    int64_t npairs = 0;
+    int64_t njobs = 0;   // result slots 0..njobs-1
+    int64_t npairs = 0;  // njobs plus the reverse halves of split pairs
+    std::vector<WfSplitDev> splits;
+    int64_t colw_entries = 0;
     const WfVariant *variant = nullptr;
     std::vector<WfPairDev> pairs;
     std::vector<WfTaskDev> tasks;

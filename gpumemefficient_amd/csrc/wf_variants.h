@@ -35,6 +35,9 @@ extern "C" hipError_t wf_launch_table(int grid, const WfKernelArgs *ka, hipStrea
 // anti-diagonal, 1 one launch per tile anti-diagonal; the full table t.
 extern "C" hipError_t wf_launch_naive(int method, int prob, int textbook, int *t, const int32_t *a,
                                       const int32_t *b, int n1, int n2, int *smax, hipStream_t stream);
+// Combine the two halves of each split pair (WfSplitDev) into res[job].
+extern "C" hipError_t wf_launch_bitpar_combine(const WfSplitDev *splits, int nsplits, const uint32_t *colw,
+                                               WfResultDev *res, hipStream_t stream);
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
 
