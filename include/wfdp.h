@@ -208,6 +208,12 @@ int wf_plan_run(wf_plan *plan, int mode);                 /* enqueue + wait */
 int wf_plan_capture(wf_plan *plan, int mode);
 int wf_plan_results(wf_plan *plan, wf_result *out, int64_t n);
 double wf_plan_kernel_ms(const wf_plan *plan);            /* last run, HIP events */
+/* engine: 0 lock-step, 1 asynchronous, 2 bit-parallel, 3 bit-parallel staged.
+ * ntasks: strips of the launch.  A bit-parallel pair of at least 1024
+ * columns in a launch with room for twice its strips runs as two column
+ * halves (the left one forward, the right one reversed) whose scores a
+ * small kernel combines; ntasks then counts the strips of both halves.
+ * WFDP_SPLIT=0 in the environment turns the split off. */
 int wf_plan_describe(const wf_plan *plan, int32_t *engine, int32_t *rows_per_lane,
                      int32_t *waves, int32_t *chunk, int64_t *ntasks, int32_t *grid);
 
