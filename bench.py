@@ -202,6 +202,12 @@ def main() -> int:
     achieved = OPS_PER_CELL[prob] * cells / (kernel_ms * 1e-3) / 1e12
     config.update({"engine": desc["engine"], "rows_per_lane": desc["rows_per_lane"], "waves": desc["waves"],
                    "chunk": desc["chunk"], "strips": desc["tasks"], "grid": desc["grid"]})
+    if desc["engine"].startswith("bitpar"):
+        # A split pair runs as a forward and a reversed column half: twice
+        # the strips of the unsplit pair.
+        strip_rows = 64 * desc["waves"] * desc["rows_per_lane"]
+        unsplit = sum((int(j.b.size) + strip_rows - 1) // strip_rows for j in jobs if j.a.size and j.b.size)
+        config["column_split"] = desc["tasks"] > unsplit
     line = {
         "metric": "GCUPS & % int-VALU peak; batched vs one-pair-per-launch; p50/p99 latency",
         "value": round(gcups, 2),

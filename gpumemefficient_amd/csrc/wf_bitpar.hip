@@ -36,6 +36,10 @@
 // from the last column's vertical differences: ED D(n2,n1) = D(0,n1) + sum of
 // Pv - Mv over the rows, LCS = number of zero bits of V, summed over the
 // lanes with atomics.
+//
+// Also here: the staged variant (a staging wave per workgroup looks up the
+// match masks for the compute waves) and the combine kernel of pairs split
+// into a forward and a reversed column half (WfSplitDev).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
