@@ -208,6 +208,29 @@ def test_table_and_last_row_agree():
                 assert t.max() == O.sw(a, b)[1]
 
 
+def test_column_split_identity():
+    """The column split the bit-parallel engine uses for single LCS/ED pairs:
+    with F the last column of the table of a[:mid] against b (the pair's own
+    row 0) and R the last column of the textbook table of the reversed
+    a[mid:] against the reversed b, the score is min_y F(y) + R(n2 - y) for
+    ED (ED.cpp's row 0 and the textbook one) and max_y for LCS."""
+    rng = np.random.default_rng(41)
+    for _ in range(150):
+        n1 = int(rng.integers(2, 48))
+        n2 = int(rng.integers(1, 70))
+        alpha = int(rng.integers(2, 6))
+        a = rng.integers(0, alpha, n1, dtype=np.int32)
+        b = rng.integers(0, alpha, n2, dtype=np.int32)
+        mid = int(rng.integers(1, n1))
+        ra, rb = a[mid:][::-1].copy(), b[::-1].copy()
+        for prob, textbook in (("ED", False), ("ED", True), ("LCS", False)):
+            f = O.table(prob, a[:mid], b, textbook=textbook)[:, mid]
+            r = O.table(prob, ra, rb, textbook=True)[:, n1 - mid]
+            sums = [int(f[y]) + int(r[n2 - y]) for y in range(n2 + 1)]
+            got = min(sums) if prob == "ED" else max(sums)
+            assert got == O.score(prob, a, b, textbook=textbook)[0], (prob, textbook, n1, n2, mid)
+
+
 def test_ed_rewrite_min_form_matches_reference_recurrence():
     """The kernels use D = min(up+1, left+1, diag+[a!=b]) in place of ED.cpp's
     `eq ? diag : 1+min(up,left,diag)` (EditDistance/ED.cpp:61-63), and
