@@ -148,3 +148,27 @@ def test_split_full_size_ed_pair(ctx):
     plan.close()
     assert d["tasks"] == 2 * _tasks_unsplit(65536)
     assert r.last == want
+
+
+@pytest.mark.parametrize("tile", [None, Tile(engine="bitpar")], ids=["staged", "plain"])
+def test_dependency_timeout_is_reported_and_the_next_run_is_exact(tile):
+    """A context whose dependency waits may last 1 us: strips that wait for
+    the strip above give up, the launch still drains (every wait is
+    bounded) and reports a device-side timeout; a context with the normal
+    limit then gives the exact score."""
+    from gpumemefficient_amd import Context, WfError
+    rng = np.random.default_rng(82)
+    a, b = _pair(rng, 20000, 9000, 20)
+    want = O.score("ED", a, b)[0]
+    with Context(0, timeout_s=1e-6) as c:
+        try:
+            r = c.align("ED", a, b, tile=tile)
+            print(f"  no wait exceeded 1 us: {r.last}", flush=True)
+            assert r.last == want
+        except WfError as e:
+            print(f"  timeout reported: {e}", flush=True)
+            from gpumemefficient_amd import _lib
+            assert e.status == _lib.WF_ERR_TIMEOUT
+    with Context(0, timeout_s=20.0) as c:
+        r = c.align("ED", a, b, tile=tile)
+        assert r.status == 0 and r.last == want
