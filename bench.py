@@ -162,7 +162,16 @@ def cpu_baseline(jobs, prob: str, budget_s: float = 20.0, min_s: float = 10.0):
     what = "the whole" if rows == b.size else f"rows 0..{rows - 1} of the"
     sample = (f"{prob} oracle (C restatement, -O3, 1 thread) on {what} {b.size}x{a.size} pair, "
               f"{reps} run(s): {cells:.3e} cells in {dt:.1f} s")
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port", "sample": sample}
+    host = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                host = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port", "sample": sample,
+            "host_cpu": host, "host_threads": os.cpu_count()}
 
 
 def main() -> int:
