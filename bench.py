@@ -128,6 +128,35 @@ def stencil_measure(ctx, exp: int = 15, steps: int = 3, warmup: int = 1, seed: i
     return out
 
 
+# The reference's only published timing of this path: LCS 32768 x 32768
+# (4 letters) through the hyperlane Share host function, 13.6486 s for 100
+# calls on a GeForce GTX 1080 Ti, every call including the table allocation,
+# the copies and the kernels (LCS/GPU_Hyperlane_Share.txt:103).
+PUBLISHED_LCS_32768_MS_PER_CALL = 136.486
+
+
+def lcs_published_measure(ctx, calls: int = 20, seed: int = 7) -> dict:
+    """The same workload through the drop-in path (host arrays in, score
+    out: symbol encoding, upload, launch and result copy per call), timed
+    like the reference's driver: wall clock of whole calls."""
+    from gpumemefficient_amd import datagen as G
+    a, b = G.uniform_pair(32768, 32768, 4, seed)
+    ctx.align("LCS", a, b)
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(calls):
+        r = ctx.align("LCS", a, b)
+        kms.append(r.kernel_ms)
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / calls
+    cells = 32768 * 32768
+    return {"workload": f"lcs_32768x32768_alpha4_seed{seed}", "calls": calls,
+            "ms_per_call_end_to_end": round(ms, 3), "gcups_end_to_end": round(cells / ms / 1e6, 1),
+            "kernel_ms": round(sum(kms) / len(kms), 3),
+            "published_ms_per_call_gtx1080ti": PUBLISHED_LCS_32768_MS_PER_CALL,
+            "speedup_vs_published": round(PUBLISHED_LCS_32768_MS_PER_CALL / ms, 1), "lcs_length": r.last}
+
+
 def time_plan(plan, steps: int, warmup: int, per_pair: bool = False):
     for _ in range(warmup):
         plan.run(per_pair=per_pair)
@@ -305,6 +334,7 @@ def main() -> int:
         serve["saturated_speedup"] = round(sat_c / sat_p, 2) if sat_p else None
         extra["serving"] = serve
         extra["stencil_32768"] = stencil_measure(ctx)
+        extra["lcs_32768_drop_in"] = lcs_published_measure(ctx)
         line["extra"] = extra
 
     if not args.no_cpu:
