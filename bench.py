@@ -215,7 +215,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(one huge pair would shard as a point-to-point boundary-row stream, see DESIGN.md)",
+              "(one huge pair would shard as bands of rows, each GPU streaming its bottom boundary row to the next)",
               file=sys.stderr)
         return 2
 
