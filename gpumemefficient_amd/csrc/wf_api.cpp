@@ -329,6 +329,13 @@ static bool split_enabled()
     const char *e = std::getenv("WFDP_SPLIT");
     return !(e && e[0] == '0');
 }
+// Byte row tables for SW pairs of at most 4 symbols (WFK_MODE_PERM4): on
+// unless WFDP_PERM=0.
+static bool perm4_enabled()
+{
+    const char *e = std::getenv("WFDP_PERM");
+    return !(e && e[0] == '0');
+}
 // Pairs narrower than this are not split.
 constexpr int64_t kSplitMinCols = 1024;
 
@@ -349,6 +356,7 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     ph->variant = v;
     const int64_t strip = 64LL * v->waves * v->rows_per_lane;
     const bool bitpar = v->engine == WF_ENGINE_BITPAR || v->engine == WF_ENGINE_BITPAR_STAGED;
+    const bool perm4 = v->engine == WF_ENGINE_ASYNC && v->rows_per_lane == 2 && perm4_enabled();
 
     // Bit-parallel pairs are split into two column halves computed at once
     // (WfSplitDev) when the launch has room for twice its strips: a single
@@ -443,6 +451,18 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if (st != WF_OK) return st;
         if (j.problem == WF_ED && (j.flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
         if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
+        // SW with at most 4 symbols on the asynchronous engine with two rows
+        // per lane: byte row tables and plain codes (WFK_MODE_PERM4).
+        if (perm4 && j.problem == WF_SW && !(mode & WFK_MODE_CMP)) {
+            bool small = true;
+            for (int64_t k = 0; small && k < j.n1; ++k) small = ph->codes[p.a_off + k] <= 12;
+            for (int64_t k = 0; small && k < j.n2; ++k) small = ph->codes[p.b_off + k] <= 12;
+            if (small) {
+                for (int64_t k = 0; k < j.n1; ++k) ph->codes[p.a_off + k] >>= 2;
+                for (int64_t k = 0; k < j.n2; ++k) ph->codes[p.b_off + k] >>= 2;
+                mode = (mode & WFK_MODE_TEXTBOOK) | WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD;
+            }
+        }
         p.mode = mode;
         // The bit-parallel kernel indexes a 32-row table with the codes.
         if (bitpar && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;

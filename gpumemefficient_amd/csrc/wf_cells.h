@@ -95,6 +95,37 @@ struct Cell<WFK_SW, CMP> {
     }
 };
 
+// SW, at most 4 symbols, two rows per lane (WFK_MODE_PERM4): the lane's two
+// row tables hold one byte per (symbol, row), s+2 as a signed byte (5 on a
+// match, -1 else), t[0] = symbols 0 and 1, t[1] = symbols 2 and 3, each as
+// (row 0, row 1); the column value is a v_perm_b32 selector that moves the
+// column symbol's two bytes to bytes 0 and 1.  One perm per step replaces
+// the two per-row field extracts, and each row's add reads its byte
+// directly.
+template <bool CMP>
+struct Cell<WFK_SWP, CMP> {
+    static constexpr int idle = 0;
+    static __device__ __forceinline__ uint32_t tables(uint32_t c0, uint32_t c1, int half)
+    {
+        // c0, c1: the rows' symbols (>= 4 for rows past the table: no match).
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t sym = 2u * (uint32_t)half + (uint32_t)k;
+            t |= (c0 == sym ? 0x05u : 0xffu) << (16 * k);
+            t |= (c1 == sym ? 0x05u : 0xffu) << (16 * k + 8);
+        }
+        return t;
+    }
+    // Selector of symbol c: bytes 2c and 2c+1 of {t[1], t[0]} to bytes 0, 1.
+    static __device__ __forceinline__ uint32_t selector(uint32_t c)
+    {
+        return 0x0c0c0000u | ((2u * c + 1u) << 8) | (2u * c);
+    }
+    // A column symbol that matches no row (selector byte 0x0d reads 0xff).
+    static constexpr uint32_t kNever = 0x0c0c0d0du;
+};
+
 template <int R>
 struct LaneState {
     int h[R];        // H(row r, last column) -- the next step's "left"
@@ -109,10 +140,20 @@ __device__ __forceinline__ void step(LaneState<R> &st, int inj, uint32_t ca, boo
     const int up0 = __builtin_amdgcn_update_dpp(inj, st.h[R - 1], kDppWaveShr1, 0xf, 0xf, false);
     int up = up0;
     int diag = st.dg;
+    uint32_t pp = 0;
+    if (P == WFK_SWP) pp = __builtin_amdgcn_perm(st.t[1], st.t[0], ca);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int left = st.h[r];
-        int h = Cell<P, CMP>::f(up, left, diag, ca, st.t[r]);
+        int h;
+        if constexpr (P == WFK_SWP) {
+            static_assert(R == 2, "WFK_SWP holds two rows per lane");
+            const int s2 = (int)(int8_t)(uint8_t)(pp >> (8 * r));
+            const int m = max(max(up, left), diag + s2);
+            h = (int)__builtin_elementwise_sub_sat((uint32_t)m, 2u);
+        } else {
+            h = Cell<P, CMP>::f(up, left, diag, ca, st.t[r]);
+        }
         if (MASKED) {
             h = valid ? h : left;
             if (TRACK) st.best = max(st.best, valid ? h : 0);

@@ -6,6 +6,8 @@
 
 // Problem ids match include/wfdp.h (WF_LCS, WF_ED, WF_SW).
 enum { WFK_LCS = 0, WFK_ED = 1, WFK_SW = 2 };
+// Kernel-internal problem id: SW in WFK_MODE_PERM4 (never stored in a pair).
+enum { WFK_SWP = 3 };
 
 // Pair mode bits.
 enum {
@@ -16,6 +18,10 @@ enum {
     // ED), or (SW) the column codes left of column 0 are a code that matches
     // no row, bits WFK_MODE_NEVER_SHIFT.. of the mode.
     WFK_MODE_FREE_HEAD = 4,
+    // SW with at most 4 symbols on the asynchronous engine, two rows per
+    // lane: the codes are plain 0..3 and a lane's row tables are bytes (see
+    // WFK_SWP in wf_cells.h).
+    WFK_MODE_PERM4 = 16,
 };
 enum { WFK_MODE_NEVER_SHIFT = 8 };
 

@@ -257,7 +257,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         }
     };
 #endif
-    constexpr bool TRACK = (P == WFK_SW);
+    constexpr bool TRACK = (P == WFK_SW || P == WFK_SWP);
     const int n1 = pd.n1, n2 = pd.n2;
     const int nsteps = n1 + 63;
     const bool has_prev = g > 0;
@@ -279,12 +279,21 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         const int64_t row0 = (int64_t)g * (W * 64 * R) + w * (64 * R) + lane * R;
         const uint8_t *bcodes = ka.codes + pd.b_off;
         LaneState<R> st;
+        if constexpr (P == WFK_SWP) {
+            const uint32_t c0 = row0 < n2 ? (uint32_t)bcodes[row0] : 4u;
+            const uint32_t c1 = row0 + 1 < n2 ? (uint32_t)bcodes[row0 + 1] : 4u;
+            st.t[0] = Cell<P, CMP>::tables(c0, c1, 0);
+            st.t[1] = Cell<P, CMP>::tables(c0, c1, 1);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int64_t y = row0 + r;
-            const bool v = y < n2;
-            st.t[r] = Cell<P, CMP>::row_entry(v ? (uint32_t)bcodes[y] : 0u, v);
-            st.h[r] = 0;
+            for (int r = 0; r < R; ++r) st.h[r] = 0;
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t y = row0 + r;
+                const bool v = y < n2;
+                st.t[r] = Cell<P, CMP>::row_entry(v ? (uint32_t)bcodes[y] : 0u, v);
+                st.h[r] = 0;
+            }
         }
         st.dg = 0;
         st.best = 0;
@@ -488,7 +497,8 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
     // row (see WFK_MODE_FREE_HEAD).  Written before the first staging
     // counter store, so every compute wave reads them after they are set.
     {
-        const uint32_t never = ((uint32_t)pd.mode >> WFK_MODE_NEVER_SHIFT) & 0x1FFu;
+        const uint32_t never =
+            (P == WFK_SWP) ? Cell<WFK_SWP, false>::kNever : ((uint32_t)pd.mode >> WFK_MODE_NEVER_SHIFT) & 0x1FFu;
         sh.aw[lane] = never;
         if (lane < kU + kB) sh.aw[AW + lane] = never;
     }
@@ -503,6 +513,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         const int x = in_next + lane;
         const bool in = x < n1;
         p.c = in ? (uint32_t)acodes[x] : 0u;
+        if (P == WFK_SWP) p.c = Cell<WFK_SWP, false>::selector(p.c & 3u);
         p.t = (has_prev && in) ? __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tag;
     };
     // Stage what poll p shows beyond in_next, within the room the compute
@@ -601,7 +612,15 @@ __global__ __launch_bounds__(64 * (W + 2)) void wf_engine_async(WfKernelArgs ka)
         case WFK_LCS * 2 + 1: async_strip<WFK_LCS, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         case WFK_ED * 2 + 0: async_strip<WFK_ED, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         case WFK_ED * 2 + 1: async_strip<WFK_ED, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
-        case WFK_SW * 2 + 0: async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
+        case WFK_SW * 2 + 0:
+            if constexpr (R == 2) {
+                if (pd.mode & WFK_MODE_PERM4) {
+                    async_strip<WFK_SWP, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
+                    break;
+                }
+            }
+            async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
+            break;
         default: async_strip<WFK_SW, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         }
         __syncthreads();

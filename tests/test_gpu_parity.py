@@ -228,3 +228,24 @@ def test_autotune_roundtrip(ctx, tmp_path):
     assert [(r.last, r.max_score) for r in plan.results()] == _expected(jobs)
     plan.close()
     tune_clear()
+
+
+def test_sw_byte_tables_match_field_tables(ctx, monkeypatch):
+    """SW pairs of at most 4 symbols on the asynchronous engine with two rows
+    per lane use byte row tables and one v_perm_b32 per step; the same
+    pairs with the 4-bit field tables (WFDP_PERM=0) and the oracle agree
+    on the last and the maximum cell: odd row counts (a padding row in the
+    last lane), pairs narrower than a wave's skew, 1 to 4 symbols."""
+    rng = np.random.default_rng(91)
+    t = Tile(engine="async", rows_per_lane=2, strip_rows=512)
+    cases = [(1, 1, 1), (5, 3, 2), (63, 129, 4), (700, 1001, 4), (2000, 515, 3), (130, 4097, 4), (3000, 3000, 1)]
+    for n1, n2, alpha in cases:
+        a, b = rand_pair(rng, n1, n2, alpha)
+        want = O.score("SW", a, b)
+        got = {}
+        for flag in ("1", "0"):
+            monkeypatch.setenv("WFDP_PERM", flag)
+            r = ctx.align("SW", a, b, tile=t)
+            got[flag] = (r.last, r.max_score)
+        print(f"  SW perm n1={n1} n2={n2} alpha={alpha}: {got} oracle {want}", flush=True)
+        assert got["1"] == got["0"] == tuple(want)
