@@ -356,7 +356,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     ph->variant = v;
     const int64_t strip = 64LL * v->waves * v->rows_per_lane;
     const bool bitpar = v->engine == WF_ENGINE_BITPAR || v->engine == WF_ENGINE_BITPAR_STAGED;
-    const bool perm4 = v->engine == WF_ENGINE_ASYNC && v->rows_per_lane == 2 && perm4_enabled();
+    const bool perm4 = (v->engine == WF_ENGINE_ASYNC || v->engine == WF_ENGINE_LOCKSTEP) && v->rows_per_lane == 2 &&
+                       perm4_enabled();
 
     // Bit-parallel pairs are split into two column halves computed at once
     // (WfSplitDev) when the launch has room for twice its strips: a single
@@ -451,8 +452,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if (st != WF_OK) return st;
         if (j.problem == WF_ED && (j.flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
         if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
-        // SW with at most 4 symbols on the asynchronous engine with two rows
-        // per lane: byte row tables and plain codes (WFK_MODE_PERM4).
+        // SW with at most 4 symbols on a cell engine with two rows per lane:
+        // byte row tables and plain codes (WFK_MODE_PERM4).
         if (perm4 && j.problem == WF_SW && !(mode & WFK_MODE_CMP)) {
             bool small = true;
             for (int64_t k = 0; small && k < j.n1; ++k) small = ph->codes[p.a_off + k] <= 12;

@@ -18,9 +18,9 @@ enum {
     // ED), or (SW) the column codes left of column 0 are a code that matches
     // no row, bits WFK_MODE_NEVER_SHIFT.. of the mode.
     WFK_MODE_FREE_HEAD = 4,
-    // SW with at most 4 symbols on the asynchronous engine, two rows per
-    // lane: the codes are plain 0..3 and a lane's row tables are bytes (see
-    // WFK_SWP in wf_cells.h).
+    // SW with at most 4 symbols on a cell engine (asynchronous or
+    // lock-step), two rows per lane: the codes are plain 0..3 and a lane's
+    // row tables are bytes (see WFK_SWP in wf_cells.h).
     WFK_MODE_PERM4 = 16,
 };
 enum { WFK_MODE_NEVER_SHIFT = 8 };

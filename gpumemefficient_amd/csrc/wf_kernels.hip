@@ -132,19 +132,33 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
 {
     constexpr int D = ChunkGeom<K>::D;
     constexpr int RS = ChunkGeom<K>::RS;
-    constexpr bool TRACK = (P == WFK_SW);
+    constexpr bool TRACK = (P == WFK_SW || P == WFK_SWP);
     const int n1 = pd.n1, n2 = pd.n2;
     const int64_t row0 = (int64_t)g * (W * 64 * R) + wave * (64 * R) + lane * R;
     const uint8_t *acodes = ka.codes + pd.a_off;
     const uint8_t *bcodes = ka.codes + pd.b_off;
+    // Column value staged for the step: the code, or (WFK_SWP) its selector.
+    auto colv = [](uint32_t c) -> uint32_t {
+        if constexpr (P == WFK_SWP) return Cell<WFK_SWP, false>::selector(c & 3u);
+        return c;
+    };
 
     LaneState<R> st;
+    if constexpr (P == WFK_SWP) {
+        const uint32_t c0 = row0 < n2 ? (uint32_t)bcodes[row0] : 4u;
+        const uint32_t c1 = row0 + 1 < n2 ? (uint32_t)bcodes[row0 + 1] : 4u;
+        st.t[0] = Cell<P, CMP>::tables(c0, c1, 0);
+        st.t[1] = Cell<P, CMP>::tables(c0, c1, 1);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int64_t y = row0 + r;
-        const bool v = y < n2;
-        st.t[r] = Cell<P, CMP>::row_entry(v ? (uint32_t)bcodes[y] : 0u, v);
-        st.h[r] = 0;
+        for (int r = 0; r < R; ++r) st.h[r] = 0;
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t y = row0 + r;
+            const bool v = y < n2;
+            st.t[r] = Cell<P, CMP>::row_entry(v ? (uint32_t)bcodes[y] : 0u, v);
+            st.h[r] = 0;
+        }
     }
     st.dg = 0;
     st.best = 0;
@@ -199,7 +213,7 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
     };
 
     if (wave == 0) {
-        const uint32_t c0 = (lane < K && lane < n1) ? acodes[lane] : 0u;
+        const uint32_t c0 = colv((lane < K && lane < n1) ? acodes[lane] : 0u);
         if (!fill(0, load_top(0), c0) && lane == 0) sh.abort_at = -1;
     }
     __syncthreads();
@@ -225,7 +239,7 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
         uint64_t top_pre = tag;
         if (wave == 0 && j + 1 < nch) {
             const int x = (j + 1) * K + lane;
-            code_pre = (lane < K && x < n1) ? acodes[x] : 0u;
+            code_pre = colv((lane < K && x < n1) ? acodes[x] : 0u);
             top_pre = load_top(j + 1);
         }
         if (cj >= 0 && cj < nch && wave < wa) {
@@ -311,7 +325,15 @@ __device__ __forceinline__ void engine_body(const WfKernelArgs &ka)
         case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_SW * 2 + 0: run_group<WFK_SW, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
+        case WFK_SW * 2 + 0:
+            if constexpr (R == 2) {
+                if (pd.mode & WFK_MODE_PERM4) {
+                    run_group<WFK_SWP, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
+                    break;
+                }
+            }
+            run_group<WFK_SW, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
+            break;
         default: run_group<WFK_SW, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         }
         __syncthreads();
