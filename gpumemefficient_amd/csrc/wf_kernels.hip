@@ -340,12 +340,20 @@ __device__ __forceinline__ void engine_body(const WfKernelArgs &ka)
     }
 }
 
-// At least 6 waves per SIMD (at most 80 VGPRs): the lock-step engine is the
-// throughput engine (batches, served mixes), where residency wins -- the
-// 4096-pair SW batch runs 1.10 ms at 80 VGPRs against 1.27 ms at the 103 the
-// compiler takes unbounded (a few spills in the edge paths).
+// Residency bound of the lock-step engine, the throughput engine (batches,
+// served mixes), where residency wins: four-wave workgroups of one or two
+// rows per lane at 8 per CU (8 waves per SIMD, at most 64 VGPRs; the
+// 4096-pair SW batch 0.99 ms against 1.03 ms at 6 per CU / 80 VGPRs, 1.09 ms
+// at 5 / 96 and 1.19 ms at 4 / 104, spill-free, so residency beats the
+// spills); other shapes at 6 per CU.
+template <int R, int W>
+constexpr int lockstep_min_blocks()
+{
+    return (W == 4 && R <= 2) ? 8 : 6;
+}
+
 template <int R, int W, int K>
-__global__ __launch_bounds__(64 * W, 6) void wf_engine(WfKernelArgs ka)
+__global__ __launch_bounds__(64 * W, (lockstep_min_blocks<R, W>())) void wf_engine(WfKernelArgs ka)
 {
     engine_body<R, W, K, false>(ka);
 }
