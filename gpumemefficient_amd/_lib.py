@@ -23,6 +23,8 @@ WF_ERR_ALPHABET = 4
 WF_ERR_TIMEOUT = 5
 WF_ERR_NOKERNEL = 6
 WF_ERR_AGAIN = 7
+WF_ERR_SCRATCH = 8
+WF_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 WF_TILE_ED_TEXTBOOK = 0x1
 WF_TILE_CACHE = 0x2
@@ -51,7 +53,7 @@ EXPORTS = [
     "wf_plan_results", "wf_plan_kernel_ms", "wf_plan_describe", "wf_plan_destroy",
     "wf_variant_count", "wf_variant_info", "wf_plan_capture", "wf_autotune", "wf_tune_load", "wf_tune_save",
     "wf_tune_clear", "wf_server_create", "wf_server_submit", "wf_server_poll", "wf_server_wait_idle",
-    "wf_server_stats", "wf_server_replay", "wf_server_destroy",
+    "wf_server_stats", "wf_server_replay", "wf_server_destroy", "wf_align_noalloc",
 ]
 
 
@@ -110,6 +112,9 @@ def load() -> C.CDLL:
     lib.wf_align.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32), C.c_int64,
                              C.POINTER(wf_tile), C.POINTER(wf_result)]
     lib.wf_align.restype = C.c_int
+    lib.wf_align_noalloc.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32),
+                                     C.c_int64, C.POINTER(wf_tile), C.POINTER(wf_result)]
+    lib.wf_align_noalloc.restype = C.c_int
     lib.wf_align_table.argtypes = [P, C.c_int, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32), C.c_int64,
                                    C.POINTER(wf_tile), C.POINTER(C.c_int32), C.POINTER(wf_result)]
     lib.wf_align_table.restype = C.c_int

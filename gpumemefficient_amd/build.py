@@ -82,7 +82,8 @@ def build_lib() -> Path:
     LIB.mkdir(exist_ok=True)
     OBJ.mkdir(exist_ok=True)
     srcs = [CSRC / "wf_engine.hip", CSRC / "wf_kernels.hip", CSRC / "wf_api.cpp", CSRC / "wf_server.cpp",
-            CSRC / "wf_tune.cpp", CSRC / "wf_naive.hip", CSRC / "wf_stencil.hip", CSRC / "wf_stencil_api.cpp", CSRC / "wf_bitpar.hip"]
+            CSRC / "wf_tune.cpp", CSRC / "wf_naive.hip", CSRC / "wf_stencil.hip", CSRC / "wf_stencil_api.cpp", CSRC / "wf_bitpar.hip",
+            CSRC / "wf_noalloc.hip"]
     srcs = [s for s in srcs if s.exists()]
     objs = [OBJ / (s.stem + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=len(srcs)) as ex:

@@ -534,22 +534,22 @@ int wf_upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
     return wf_upload_plan_async(ctx, ph, ar, ctx->stream, true);
 }
 
-int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync)
+void wf_plan_bytes(const WfPlanHost *ph, WfPlanBytes *b)
 {
-    (void)ctx;
+    b->codes = ph->codes.size();
+    b->pairs = std::max<size_t>(1, ph->pairs.size()) * sizeof(WfPairDev);
+    b->tasks = std::max<size_t>(1, ph->tasks.size()) * sizeof(WfTaskDev);
+    b->bnd = (size_t)std::max<int64_t>(1, ph->bnd_ints) * sizeof(uint64_t);
+    b->res = (size_t)std::max<int64_t>(1, ph->npairs) * sizeof(WfResultDev);
+    b->counters = (size_t)(ph->npairs + 1) * sizeof(int32_t);
+    b->epoch = sizeof(uint32_t);
+    b->colw = (size_t)std::max<int64_t>(1, ph->colw_entries) * 2 * sizeof(uint32_t);
+    b->splits = std::max<size_t>(1, ph->splits.size()) * sizeof(WfSplitDev);
+}
+
+int wf_copy_plan(WfPlanHost *ph, WfArenas *ar, hipStream_t stream)
+{
     const size_t ntasks = ph->tasks.size();
-    int st;
-    if ((st = ar->codes.ensure(ph->codes.size())) != WF_OK) return st;
-    if ((st = ar->pairs.ensure(std::max<size_t>(1, ph->pairs.size()) * sizeof(WfPairDev))) != WF_OK) return st;
-    if ((st = ar->tasks.ensure(std::max<size_t>(1, ntasks) * sizeof(WfTaskDev))) != WF_OK) return st;
-    if ((st = ar->bnd.ensure(std::max<int64_t>(1, ph->bnd_ints) * sizeof(uint64_t), true)) != WF_OK) return st;
-    if ((st = ar->res.ensure(std::max<int64_t>(1, ph->npairs) * sizeof(WfResultDev))) != WF_OK) return st;
-    if ((st = ar->counters.ensure((size_t)(ph->npairs + 1) * sizeof(int32_t))) != WF_OK) return st;
-    // Launch epoch: zero on allocation, so it never matches a zeroed boundary word.
-    if ((st = ar->epoch.ensure(sizeof(uint32_t), true)) != WF_OK) return st;
-    if ((st = ar->colw.ensure((size_t)std::max<int64_t>(1, ph->colw_entries) * 2 * sizeof(uint32_t))) != WF_OK)
-        return st;
-    if ((st = ar->splits.ensure(std::max<size_t>(1, ph->splits.size()) * sizeof(WfSplitDev))) != WF_OK) return st;
     if (hipMemcpyAsync(ar->codes.ptr, ph->codes.data(), ph->codes.size(), hipMemcpyHostToDevice,
                        stream) != hipSuccess)
         return WF_ERR_DEVICE;
@@ -565,6 +565,26 @@ int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t 
         hipMemcpyAsync(ar->tasks.ptr, ph->tasks.data(), ntasks * sizeof(WfTaskDev),
                        hipMemcpyHostToDevice, stream) != hipSuccess)
         return WF_ERR_DEVICE;
+    return WF_OK;
+}
+
+int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync)
+{
+    (void)ctx;
+    WfPlanBytes nb;
+    wf_plan_bytes(ph, &nb);
+    int st;
+    if ((st = ar->codes.ensure(nb.codes)) != WF_OK) return st;
+    if ((st = ar->pairs.ensure(nb.pairs)) != WF_OK) return st;
+    if ((st = ar->tasks.ensure(nb.tasks)) != WF_OK) return st;
+    if ((st = ar->bnd.ensure(nb.bnd, true)) != WF_OK) return st;
+    if ((st = ar->res.ensure(nb.res)) != WF_OK) return st;
+    if ((st = ar->counters.ensure(nb.counters)) != WF_OK) return st;
+    // Launch epoch: zero on allocation, so it never matches a zeroed boundary word.
+    if ((st = ar->epoch.ensure(nb.epoch, true)) != WF_OK) return st;
+    if ((st = ar->colw.ensure(nb.colw)) != WF_OK) return st;
+    if ((st = ar->splits.ensure(nb.splits)) != WF_OK) return st;
+    if ((st = wf_copy_plan(ph, ar, stream)) != WF_OK) return st;
     if (sync && hipStreamSynchronize(stream) != hipSuccess) return WF_ERR_DEVICE;
     return WF_OK;
 }
@@ -643,8 +663,7 @@ int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int max
 
 // Run a plan (directly, or by replaying its captured graph) and time the
 // device work with HIP events on the context stream.
-static int run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *kernel_ms,
-                    hipGraphExec_t exec = nullptr)
+int wf_run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *kernel_ms, hipGraphExec_t exec)
 {
     const int maxg = wf_max_resident_groups(ctx, ph->variant);
     if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
@@ -722,6 +741,7 @@ const char *wf_strerror(int status)
     case WF_ERR_TIMEOUT: return "device-side dependency wait timed out";
     case WF_ERR_NOKERNEL: return "no compiled kernel for the requested tile";
     case WF_ERR_AGAIN: return "queue full, retry";
+    case WF_ERR_SCRATCH: return "the pair does not fit the fixed scratch area of wf_align_noalloc (64 MiB)";
     default: return "unknown error";
     }
 }
@@ -744,6 +764,9 @@ int wf_ctx_create(int device, wf_ctx **out)
         delete c;
         return WF_ERR_DEVICE;
     }
+    // Set up this device's static scratch area of wf_align_noalloc (once per
+    // device); a failure here is reported by that entry point's own call.
+    (void)wf_noalloc_setup(device, c->stream, nullptr);
     *out = c;
     return WF_OK;
 }
@@ -795,7 +818,7 @@ int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile
     if (st != WF_OK) return st;
     if ((st = wf_upload_plan(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
     double ms = 0.0;
-    if ((st = run_plan(ctx, &ph, &ctx->scratch, WF_RUN_PERSISTENT, &ms)) != WF_OK) return st;
+    if ((st = wf_run_plan(ctx, &ph, &ctx->scratch, WF_RUN_PERSISTENT, &ms, nullptr)) != WF_OK) return st;
     return wf_download_results(ctx, &ph, &ctx->scratch, ms, out, njobs);
 }
 
@@ -930,7 +953,7 @@ int wf_plan_run(wf_plan *plan, int mode)
 {
     if (!plan || (mode != WF_RUN_PERSISTENT && mode != WF_RUN_PER_PAIR)) return WF_ERR_INVALID;
     if (hipSetDevice(plan->ctx->device) != hipSuccess) return WF_ERR_DEVICE;
-    const int st = run_plan(plan->ctx, &plan->host, &plan->own, mode, &plan->kernel_ms, plan->exec[mode]);
+    const int st = wf_run_plan(plan->ctx, &plan->host, &plan->own, mode, &plan->kernel_ms, plan->exec[mode]);
     plan->ran = (st == WF_OK);
     return st;
 }

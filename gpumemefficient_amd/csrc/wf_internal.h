@@ -105,10 +105,24 @@ int wf_default_waves(int64_t max_rows, int64_t total_cells, int npairs);
 int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, int npairs);
 int wf_build_plan(struct wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                   WfPlanHost *ph);
+// Device bytes of every buffer a plan needs.
+struct WfPlanBytes {
+    size_t codes, pairs, tasks, bnd, res, counters, epoch, colw, splits;
+};
+void wf_plan_bytes(const WfPlanHost *ph, WfPlanBytes *b);
+// Copy the plan's codes and descriptors into buffers that already hold
+// wf_plan_bytes() (asynchronous on `stream`; the host vectors must outlive it).
+int wf_copy_plan(WfPlanHost *ph, WfArenas *ar, hipStream_t stream);
 int wf_upload_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar);
 int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync);
 int wf_max_resident_groups(wf_ctx *ctx, const WfVariant *v);
 int wf_enqueue_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, int maxg, hipStream_t stream);
+// Reset + engine launch(es) (or a captured graph), timed with HIP events on
+// the context stream; returns after the stream has drained.
+int wf_run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *kernel_ms, hipGraphExec_t exec);
+// wf_noalloc.hip: device address of this device's static scratch array
+// (looked up and cleared on the first call per device).
+int wf_noalloc_setup(int device, hipStream_t stream, uint8_t **out);
 int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms, wf_result *out, int64_t n);
 
 #endif

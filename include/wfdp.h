@@ -55,7 +55,8 @@ enum {
     WF_ERR_ALPHABET = 4,  /* more than 256 distinct symbols in one pair */
     WF_ERR_TIMEOUT = 5,   /* a device-side dependency wait exceeded its limit */
     WF_ERR_NOKERNEL = 6,  /* no compiled kernel matches the requested tile */
-    WF_ERR_AGAIN = 7      /* server: queue full, retry later */
+    WF_ERR_AGAIN = 7,     /* server: queue full, retry later */
+    WF_ERR_SCRATCH = 8    /* wf_align_noalloc: the pair needs more than its fixed scratch area */
 };
 
 /* Tile selection.  strip_rows = 64 * waves * rows_per_lane rows are swept by
@@ -133,7 +134,23 @@ int wf_ctx_set_timeout(wf_ctx *ctx, double seconds);
 int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
              int64_t n2, const wf_tile *tile, wf_result *out);
 
-/* Many pairs, synchronous, one persistent launch. */
+/* wf_align() without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that
+ * must not call hipMalloc, such as tight serving loops, harnesses that forbid
+ * it, work recorded into a graph).  Every device buffer the call needs
+ * (symbol codes, descriptors, strip boundaries, results, counters) comes from
+ * one static device array of WF_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; the caller passes the same host inputs and output as to
+ * wf_align().  Pairs whose strip boundaries or codes do not fit return
+ * WF_ERR_SCRATCH (roughly: (n2 / strip_rows - 1) * n1 * 8 bytes of boundary
+ * rows within 56 MiB, and n1 + n2 bytes of codes within 8 MiB; 16384 x 16384
+ * SW fits).  All calls on one GPU share that GPU's array, and calls that
+ * overlap on one GPU (several threads or contexts) race: the caller
+ * serialises them. */
+#define WF_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int wf_align_noalloc(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b,
+                     int64_t n2, const wf_tile *tile, wf_result *out);
+
 /* Full DP table of one pair (the reference drivers' DEBUG dump,
  * LCS/GPU_Hyperlane_Share.cpp:109-131): fills table[(y-1)*n1 + (x-1)] = H(y, x)
  * for 1 <= y <= n2, 1 <= x <= n1 in the reference's values (LCS and SW cell
