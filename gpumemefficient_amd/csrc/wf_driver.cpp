@@ -142,7 +142,9 @@ int main(int argc, char **argv)
     for (int i = 0; i < itr; i++) {
 #if defined(WF_DRIVER_PROBLEM_LCS)
 #if defined(WF_DRIVER_STYLE_SHFL)
-        score = LCS(n1, n2, arr1.data(), arr2.data());
+        // LCS-SHFL/GPU_Hyperlane_Share.cpp:100 and GPU_Hyperlane_Cache.cpp:100
+        // call the 7-argument form.
+        score = LCS(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr);
 #else
         score = tiles ? LCS(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr, tX, tY)
                       : LCS(n1, n2, arr1.data(), arr2.data(), paddX, paddY, nullptr);

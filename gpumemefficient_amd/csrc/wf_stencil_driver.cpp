@@ -54,8 +54,16 @@ int main(int argc, char **argv)
     int nn1 = 0, nn2 = 0, tX = 0, tY = 0, itr = 100;
 #if defined(WF_SDRIVER_SHARE5)
     if (argc != 6) {
-        cout << "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one total iteration step."
-             << endl;
+        // SAT/GPU_Hyperlane_Share.cpp:46 and
+        // 2D-SOR/nested_loop/GPU_Hyperlane_Share.cpp:46 word this line
+        // differently.
+        if (std::string(WF_SDRIVER_NAME) == "SAT")
+            cout << "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one iteration step."
+                 << endl;
+        else
+            cout << "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one total iteration "
+                    "step."
+                 << endl;
         exit(EXIT_FAILURE);
     }
     nn1 = atoi(argv[1]);

@@ -4,8 +4,11 @@
  *
  * Link libwfdp_compat_share.so or libwfdp_compat_cache.so (plus libwfdp.so)
  * in place of the reference's .cu objects and its driver .cpp files build
- * unchanged.  The two libraries differ only in the tile policy of the calls
- * that carry no tile arguments:
+ * unchanged.  (The SmithWaterman and EditDistance Cache drivers include the
+ * Share header, GPU_Hyperlane_Cache.cpp:7, whose 9-argument declaration does
+ * not match their 7-argument call at :101; compile them with
+ * -include wfdp_compat.h, which declares both.)  The two libraries differ
+ * only in the tile policy of the calls that carry no tile arguments:
  *   share: LCS 32x256 (LCS/GPU_Hyperlane_Share.cu:703-704)
  *   cache: 256x1024, strip boundaries through L2 only
  *          (LCS/GPU_Hyperlane_Cache.cu:254-255 and the SW/ED Cache hosts)
@@ -22,9 +25,11 @@
 #ifndef WFDP_COMPAT_H
 #define WFDP_COMPAT_H
 
-/* LCS/GPU_Hyperlane_Share.h:5, LCS/GPU_Hyperlane_Cache.h:5 */
+/* LCS/GPU_Hyperlane_Share.h:5, LCS/GPU_Hyperlane_Cache.h:5,
+ * LCS-SHFL/GPU_Hyperlane_Share.h:5, LCS-SHFL/GPU_Hyperlane_Cache.h:5 */
 int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
-/* LCS-SHFL/GPU_Hyperlane_Share.h:5, LCS-SHFL/GPU_Hyperlane_Cache.h:5 */
+/* LCS/GPU.h:5, LCS-SHFL/GPU.h:5 (the naive programs' form; in the share and
+ * cache libraries it runs the hyperlane path with the library's tile) */
 int LCS(int n1, int n2, int *arr1, int *arr2);
 /* SmithWaterman/GPU_Hyperlane_Share.h:5 */
 int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);

@@ -92,17 +92,68 @@ def test_invalid_arguments_rejected_without_device():
     assert lib.wf_stencil_device(None, 1, 1, 1, None, 4, None, ctypes.byref(out)) == _lib.WF_ERR_INVALID
 
 
+# The usage line each reference driver prints for a wrong argc, verbatim.
+USAGE_TWO = "Incorrect Input Parameters. Must be two string sizes."
+USAGE = {
+    # SmithWaterman/GPU_Hyperlane_Share.cpp:51 (its own spacing)
+    ("SmithWaterman", "GPU_Hyperlane_Share"):
+        "Incorrect Input Parameters. Must be two string sizes ,two tile sizes and one total iteration step.",
+    # EditDistance/GPU_Hyperlane_Share.cpp:51
+    ("EditDistance", "GPU_Hyperlane_Share"):
+        "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one total iteration step.",
+    # SAT/GPU_Hyperlane_Share.cpp:46
+    ("SAT", "GPU_Hyperlane_Share"):
+        "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one iteration step.",
+    # 2D-SOR/nested_loop/GPU_Hyperlane_Share.cpp:46
+    ("2D-SOR", "GPU_Hyperlane_Share"):
+        "Incorrect Input Parameters. Must be two string sizes, two tile sizes, and one total iteration step.",
+}
+
+
 @pytest.mark.parametrize("pdir,name", DRIVERS)
 def test_driver_argv_errors(tmp_path, pdir, name):
+    """Wrong argc and a missing input file: exit code 1 and the reference's
+    lines, whole (LCS/GPU_Hyperlane_Share.cpp:52, :19 and the other drivers'
+    counterparts)."""
     exe = BIN / pdir / name
     r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 1
-    assert r.stdout.startswith("Incorrect Input Parameters.")
-    args = ["9", "9", "32", "128", "1"] if (name.endswith("Share") and pdir in ("SmithWaterman", "EditDistance",
-                                                                               "SAT", "2D-SOR")) else ["9", "9"]
+    assert r.stdout == USAGE.get((pdir, name), USAGE_TWO) + "\n"
+    args = ["9", "9", "32", "128", "1"] if (pdir, name) in USAGE else ["9", "9"]
     r = subprocess.run([str(exe), *args], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 1
-    assert "ERROR: Input file cannot be opened!" in r.stdout
+    lines = r.stdout.splitlines()
+    assert lines[-1] == "ERROR: Input file cannot be opened!", lines
+
+
+REF = Path("/root/reference")
+REF_DRIVERS = [(d, f) for d in ("LCS", "SmithWaterman", "EditDistance", "LCS-SHFL") for f in ("Share", "Cache")]
+
+
+@pytest.mark.skipif(not REF.is_dir(), reason="the reference tree is not mounted on this host")
+@pytest.mark.parametrize("pdir,kind", REF_DRIVERS)
+def test_reference_driver_sources_link_against_compat_libs(tmp_path, pdir, kind):
+    """The reference's own driver sources, compiled where they lie and linked
+    against libwfdp_compat_{share,cache}.so instead of the reference's .cu
+    objects.  The SmithWaterman and EditDistance Cache drivers include the
+    Share header (GPU_Hyperlane_Cache.cpp:7) but call the 7-argument form
+    (:101), so every driver is compiled with wfdp_compat.h force-included,
+    which declares both forms.  Link only: running needs a GPU."""
+    src = REF / pdir / f"GPU_Hyperlane_{kind}.cpp"
+    exe = tmp_path / f"{pdir}_{kind}"
+    cmd = ["g++", "-std=c++11", "-O1", "-w", "-include", str(ROOT / "include" / "wfdp_compat.h"),
+           f"-I{REF / pdir}", str(src), "-o", str(exe), f"-L{LIBDIR}", f"-lwfdp_compat_{kind.lower()}", "-lwfdp",
+           f"-Wl,-rpath,{LIBDIR}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    undefined = subprocess.run(["nm", "-u", str(exe)], capture_output=True, text=True, check=True).stdout
+    host_fn = {"LCS": "_Z3LCSiiPiS_iiS_", "LCS-SHFL": "_Z3LCSiiPiS_iiS_",
+               "SmithWaterman": "_Z2SWiiPiS_iiS_ii" if kind == "Share" else "_Z2SWiiPiS_iiS_",
+               "EditDistance": "_Z2EDiiPiS_iiS_ii" if kind == "Share" else "_Z2EDiiPiS_iiS_"}[pdir]
+    assert host_fn in undefined
+    exported = subprocess.run(["nm", "-D", "--defined-only", str(LIBDIR / f"libwfdp_compat_{kind.lower()}.so")],
+                              capture_output=True, text=True, check=True).stdout
+    assert host_fn in exported
 
 
 def test_datafile_roundtrip(tmp_path):
