@@ -395,7 +395,13 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
                 st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
                 need_in, need_out, wait_next, publish);
         if (aborted) {
-            if (lane == 0) __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // Mark the pair failed: in a pair's last strip no output wave
+            // exists, and an input wave that has staged every column is gone,
+            // so nobody else would.
+            if (lane == 0) {
+                atomicOr(&ka.res[pair].status, 1);
+                __hip_atomic_store(&sh.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         } else {
             const int64_t ylast = (int64_t)n2 - 1;
             if (ylast >= row0 && ylast < row0 + R) {

@@ -89,6 +89,33 @@ static int64_t request_bytes(int64_t n1, int64_t n2, int strip_rows)
            (int64_t)sizeof(WfPairDev) + strips * (int64_t)sizeof(WfTaskDev) + (int64_t)sizeof(WfResultDev);
 }
 
+// At most 256 distinct symbols: every engine encodes a pair's symbols to
+// one-byte codes (wf_encode_pair).  Checked at submission so that one bad
+// request never reaches a launch shared with others.
+static bool alphabet_fits(const wf_job *job)
+{
+    bool small[256] = {};
+    std::vector<int32_t> other;
+    int distinct = 0;
+    auto see = [&](int32_t v) {
+        if ((uint32_t)v < 256u) {
+            if (!small[v]) {
+                small[v] = true;
+                ++distinct;
+            }
+        } else {
+            other.push_back(v);
+        }
+    };
+    for (int64_t i = 0; i < job->n1; ++i) see(job->a[i]);
+    for (int64_t i = 0; i < job->n2; ++i) see(job->b[i]);
+    if (!other.empty()) {
+        std::sort(other.begin(), other.end());
+        distinct += (int)(std::unique(other.begin(), other.end()) - other.begin());
+    }
+    return distinct <= 256;
+}
+
 static void finish_slot(wf_server *s, Slot &sl)
 {
     std::vector<wf_completion> out;
@@ -237,8 +264,11 @@ static void scheduler(wf_server *s)
             for (Request &r : batch) r.t_start = t;
             const int st = launch_slot(s, sl, std::move(batch));
             if (st != WF_OK) {
+                // The launch's requests fail with its status; only a HIP
+                // runtime error stops the server (requests are validated at
+                // submission, so a plan cannot fail for one of them).
                 std::lock_guard<std::mutex> lk(s->mu);
-                s->error = st;
+                if (st == WF_ERR_DEVICE) s->error = st;
                 for (Request &r : sl.reqs) {
                     wf_completion c{};
                     c.ticket = r.ticket;
@@ -309,8 +339,9 @@ int wf_server_create(int device, const wf_server_opts *opts, wf_server **out)
 int wf_server_submit(wf_server *s, const wf_job *job, uint64_t *ticket)
 {
     if (!s || !job || job->problem < WF_LCS || job->problem > WF_SW || job->n1 < 0 || job->n2 < 0 ||
-        (job->n1 > 0 && !job->a) || (job->n2 > 0 && !job->b))
+        (job->n1 > 0 && !job->a) || (job->n2 > 0 && !job->b) || job->n1 > (1LL << 30) || job->n2 > (1LL << 30))
         return WF_ERR_INVALID;
+    if (!alphabet_fits(job)) return WF_ERR_ALPHABET;
     Request r;
     r.problem = job->problem;
     r.flags = job->flags;

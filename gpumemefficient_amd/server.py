@@ -50,10 +50,18 @@ class Server:
         return int(t.value)
 
     def poll(self, max_n: int = 4096) -> list[Completion]:
+        """Completions since the last poll.  After a device error the
+        completions taken off the queue ride on the raised WfError
+        (``.completions``) instead of being lost."""
         buf = (wf_completion * max_n)()
         n = C.c_int()
-        check(self._lib.wf_server_poll(self._h, buf, max_n, C.byref(n)), "wf_server_poll")
-        return [_comp(buf[i]) for i in range(n.value)]
+        st = self._lib.wf_server_poll(self._h, buf, max_n, C.byref(n))
+        comps = [_comp(buf[i]) for i in range(n.value)]
+        if st != _lib.WF_OK:
+            err = _lib.WfError(st, "wf_server_poll")
+            err.completions = comps
+            raise err
+        return comps
 
     def wait_idle(self, timeout_s: float = 60.0) -> None:
         check(self._lib.wf_server_wait_idle(self._h, timeout_s), "wf_server_wait_idle")

@@ -6,9 +6,10 @@
  * from its driver main()s (all paths relative to the reference tree):
  *
  *   int LCS(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
- *       LCS/GPU_Hyperlane_Share.h:5, LCS/GPU_Hyperlane_Cache.h:5
- *   int LCS(int n1, int n2, int *arr1, int *arr2);
+ *       LCS/GPU_Hyperlane_Share.h:5, LCS/GPU_Hyperlane_Cache.h:5,
  *       LCS-SHFL/GPU_Hyperlane_Share.h:5, LCS-SHFL/GPU_Hyperlane_Cache.h:5
+ *   int LCS(int n1, int n2, int *arr1, int *arr2);
+ *       LCS/GPU.h:5, LCS-SHFL/GPU.h:5 (the naive programs)
  *   int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table, int tX, int tY);
  *       SmithWaterman/GPU_Hyperlane_Share.h:5
  *   int SW(int n1, int n2, int *arr1, int *arr2, int paddX, int paddY, int *table);
@@ -291,9 +292,14 @@ typedef struct {
 } wf_server_stats_t;
 
 int wf_server_create(int device, const wf_server_opts *opts, wf_server **out);
-/* Copies the job's sequences; returns at once with a ticket. */
+/* Copies the job's sequences; returns at once with a ticket.  A request that
+ * no launch could run is refused here (WF_ERR_INVALID, WF_ERR_ALPHABET for
+ * more than 256 distinct symbols) and never joins a batch. */
 int wf_server_submit(wf_server *s, const wf_job *job, uint64_t *ticket);
-/* Completed requests since the last poll (up to max). */
+/* Completed requests since the last poll (up to max); *n is set and the
+ * completions are returned even when the status is an error.  Per-request
+ * failures are in each completion's status; the function's status is
+ * WF_ERR_DEVICE only after a HIP runtime error, which stops the server. */
 int wf_server_poll(wf_server *s, wf_completion *out, int max, int *n);
 int wf_server_wait_idle(wf_server *s, double timeout_s);
 int wf_server_stats(wf_server *s, wf_server_stats_t *st);

@@ -150,25 +150,47 @@ def test_split_full_size_ed_pair(ctx):
     assert r.last == want
 
 
-@pytest.mark.parametrize("tile", [None, Tile(engine="bitpar")], ids=["staged", "plain"])
-def test_dependency_timeout_is_reported_and_the_next_run_is_exact(tile):
-    """A context whose dependency waits may last 1 us: strips that wait for
-    the strip above give up, the launch still drains (every wait is
-    bounded) and reports a device-side timeout; a context with the normal
-    limit then gives the exact score."""
+TIMEOUT_CASES = [
+    # (problem, n1, n2, alphabet, tile): the two bit-parallel engines on a
+    # multi-strip ED pair; the asynchronous and lock-step cell engines on SW
+    # pairs, one of them a single strip (no output wave, and an input wave
+    # that stages all 200 columns at once and leaves).
+    ("ED", 20000, 9000, 20, None),
+    ("ED", 20000, 9000, 20, Tile(engine="bitpar")),
+    ("SW", 200, 300, 4, Tile(engine="async")),
+    ("SW", 3000, 2000, 4, Tile(engine="async")),
+    ("SW", 200, 300, 4, Tile(engine="lockstep")),
+    ("SW", 3000, 2000, 4, Tile(engine="lockstep")),
+]
+
+
+@pytest.mark.parametrize("prob,n1,n2,alpha,tile", TIMEOUT_CASES,
+                         ids=["ed-staged", "ed-plain", "sw-async-1strip", "sw-async", "sw-lockstep-1strip",
+                              "sw-lockstep"])
+def test_dependency_timeout_is_reported_and_the_next_run_is_exact(prob, n1, n2, alpha, tile):
+    """A context whose dependency waits may last 1 us: waves and strips
+    that wait for the ones above give up, the launch still drains (every
+    wait is bounded) and either reports a device-side timeout or, if no
+    wait ran out, returns the exact score -- never a wrong score with
+    status ok.  A context with the normal limit then gives the exact
+    score."""
     from gpumemefficient_amd import Context, WfError
     rng = np.random.default_rng(82)
-    a, b = _pair(rng, 20000, 9000, 20)
-    want = O.score("ED", a, b)[0]
+    a, b = _pair(rng, n1, n2, alpha)
+    want = O.score(prob, a, b)
     with Context(0, timeout_s=1e-6) as c:
         try:
-            r = c.align("ED", a, b, tile=tile)
+            r = c.align(prob, a, b, tile=tile)
             print(f"  no wait exceeded 1 us: {r.last}", flush=True)
-            assert r.last == want
+            assert r.last == want[0]
+            if prob == "SW":
+                assert r.max_score == want[1]
         except WfError as e:
             print(f"  timeout reported: {e}", flush=True)
             from gpumemefficient_amd import _lib
             assert e.status == _lib.WF_ERR_TIMEOUT
     with Context(0, timeout_s=20.0) as c:
-        r = c.align("ED", a, b, tile=tile)
-        assert r.status == 0 and r.last == want
+        r = c.align(prob, a, b, tile=tile)
+        assert r.status == 0 and r.last == want[0]
+        if prob == "SW":
+            assert r.max_score == want[1]

@@ -123,3 +123,28 @@ def test_server_per_request_baseline_matches():
     assert st["launches"] == len(jobs)
     for c, j in zip(comps, jobs):
         assert (c.last, c.max_score) == _want(j)
+
+
+@pytest.mark.gpu
+def test_server_refuses_a_bad_request_and_serves_the_rest():
+    """A request of 300 distinct symbols (more than the one-byte codes hold)
+    is refused at submission with WF_ERR_ALPHABET; the good requests around
+    it complete exactly, and the server keeps serving afterwards."""
+    from gpumemefficient_amd import WfError
+    jobs = _jobs(34, 40, 50, 2000)
+    want = [_want(j) for j in jobs]
+    bad = Job("SW", np.arange(300, dtype=np.int32), np.arange(300, dtype=np.int32)[::-1].copy())
+    with Server(0, streams=2) as srv:
+        tickets = [srv.submit(j) for j in jobs[:20]]
+        with pytest.raises(WfError) as ei:
+            srv.submit(bad)
+        assert ei.value.status == _lib.WF_ERR_ALPHABET
+        tickets += [srv.submit(j) for j in jobs[20:]]
+        srv.wait_idle(120)
+        got = {c.ticket: c for c in srv.poll()}
+        again = srv.submit(jobs[0])
+        srv.wait_idle(60)
+        more = srv.poll()
+    for t, w in zip(tickets, want):
+        assert got[t].status == 0 and (got[t].last, got[t].max_score) == w
+    assert len(more) == 1 and more[0].ticket == again and (more[0].last, more[0].max_score) == want[0]
