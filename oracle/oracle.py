@@ -12,6 +12,10 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   ed   golden vectors from ED.cpp (including the row-0 quirk), brute force of
        the textbook Levenshtein distance in textbook mode, closed forms
   table / last_row  checked against lcs/sw/ed and the reference full tables
+  blocked  (the all-cores anti-diagonal variant) checked against lcs/sw/ed
+       on random non-square pairs, tiles that do not divide the sizes and
+       thread counts from 1 to 9, and through the CPU drop-in programs
+       (bin/LCS, bin/SW, bin/ED) against the reference's own outputs
   stencil  SAT: the closed form (2-D prefix sums of the interior plus the
        boundary row and column, mod 2^32); SOR: hand-computed known answers
        (which neighbours are new, which old; truncation of negative sums),
@@ -35,7 +39,7 @@ _lib = None
 
 
 def build() -> None:
-    subprocess.run(["make", "-s", "-C", str(HERE), "oracle"], check=True)
+    subprocess.run(["make", "-s", "-C", str(HERE), "oracle", "cpu"], check=True)
 
 
 def load() -> C.CDLL:
@@ -58,6 +62,8 @@ def load() -> C.CDLL:
     lib.oracle_last_row.restype = C.c_int
     lib.oracle_stencil.argtypes = [C.c_int, C.c_int64, C.c_int64, I32P, C.c_int64]
     lib.oracle_stencil.restype = None
+    lib.oracle_blocked.argtypes = [C.c_int, I32P, C.c_int64, I32P, C.c_int64, C.c_int, C.c_int, C.c_int64, I32P]
+    lib.oracle_blocked.restype = C.c_int32
     _lib = lib
     return lib
 
@@ -100,6 +106,16 @@ def score(problem, a, b, textbook: bool = False) -> tuple[int, int]:
     if p == SW:
         return sw(a, b)
     return ed(a, b, textbook), 0
+
+
+def blocked(problem, a, b, threads: int, tile: int = 256, textbook: bool = False) -> tuple[int, int]:
+    """(last, max) computed by `threads` threads over anti-diagonals of
+    tile x tile blocks."""
+    a, b = _a(a), _a(b)
+    m = C.c_int32(0)
+    last = load().oracle_blocked(problem_id(problem), _p(a), a.size, _p(b), b.size, 1 if textbook else 0,
+                                 int(threads), int(tile), C.byref(m))
+    return int(last), int(m.value)
 
 
 def table(problem, a, b, textbook: bool = False) -> np.ndarray:

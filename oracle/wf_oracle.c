@@ -256,3 +256,160 @@ void oracle_stencil(int op, int64_t n1, int64_t n2, int32_t *t, int64_t ld)
         }
     }
 }
+
+/*
+ * All-cores variant of oracle_lcs / oracle_sw / oracle_ed: the table is cut
+ * into tiles of `tile` x `tile` cells and the tiles of one anti-diagonal of
+ * tiles are computed in parallel by `threads` POSIX threads, with a barrier
+ * between anti-diagonals (the CPU form of the reference's wavefront).  Each
+ * tile reads its top row from top[] (the bottom row of the tile above), its
+ * left column from left[] (the right column of the tile to its left) and
+ * its top-left corner from corner[], and writes its own bottom row, right
+ * column and bottom-right corner back.  Same recurrences and boundaries as
+ * the rolling-row functions above, so the results are identical; *max_out
+ * gets the SW maximum.  Returns H(n2, n1).
+ */
+#include <pthread.h>
+
+typedef struct {
+    int problem, textbook, threads;
+    const int32_t *a, *b;
+    int64_t n1, n2, tile, nI, nJ;
+    int32_t *top, *left, *corner; /* corner[(I) * (nJ + 1) + J] = H(I*tile, J*tile) */
+    int32_t *best;                /* per thread */
+    pthread_barrier_t bar;
+    int64_t next;                 /* tile index within the current anti-diagonal */
+    pthread_mutex_t mu;
+} Blocked;
+
+typedef struct {
+    Blocked *B;
+    int id;
+} BlockedArg;
+
+static void blocked_tile(Blocked *B, int64_t I, int64_t J, int32_t *row, int32_t *best)
+{
+    const int64_t T = B->tile;
+    const int64_t y0 = I * T + 1, y1 = (I + 1) * T < B->n2 ? (I + 1) * T : B->n2;
+    const int64_t x0 = J * T + 1, x1 = (J + 1) * T < B->n1 ? (J + 1) * T : B->n1;
+    const int64_t w = x1 - x0 + 1;
+    /* row[0] = H(y, x0 - 1), row[1..w] = H(y, x0..x1) of the row above */
+    memcpy(row + 1, B->top + x0, (size_t)w * sizeof(int32_t));
+    int32_t diag0 = B->corner[I * (B->nJ + 1) + J];
+    int32_t m = *best;
+    for (int64_t y = y0; y <= y1; ++y) {
+        const int32_t by = B->b[y - 1];
+        const int32_t lf = B->left[y];
+        int32_t diag = diag0, left = lf;
+        for (int64_t k = 1; k <= w; ++k) {
+            const int32_t up = row[k];
+            const int eq = B->a[x0 + k - 2] == by;
+            int32_t v;
+            if (B->problem == ORACLE_LCS) {
+                v = eq ? diag + 1 : imax(left, up);
+            } else if (B->problem == ORACLE_SW) {
+                v = imax(left - 2, imax(up - 2, imax(diag + (eq ? 3 : -3), 0)));
+                m = imax(m, v);
+            } else {
+                v = imin(up + 1, imin(left + 1, diag + 1));
+                if (eq) v = diag;
+            }
+            diag = up;
+            row[k] = v;
+            left = v;
+        }
+        diag0 = lf;
+        B->left[y] = left; /* H(y, x1): the left column of tile (I, J + 1) */
+    }
+    memcpy(B->top + x0, row + 1, (size_t)w * sizeof(int32_t));
+    B->corner[(I + 1) * (B->nJ + 1) + J + 1] = row[w];
+    *best = m;
+}
+
+static void *blocked_worker(void *p)
+{
+    BlockedArg *arg = (BlockedArg *)p;
+    Blocked *B = arg->B;
+    int32_t *row = (int32_t *)malloc(((size_t)B->tile + 1) * sizeof(int32_t));
+    int32_t best = 0;
+    for (int64_t d = 0; d <= B->nI + B->nJ - 2; ++d) {
+        const int64_t ilo = d - (B->nJ - 1) > 0 ? d - (B->nJ - 1) : 0;
+        const int64_t ihi = d < B->nI - 1 ? d : B->nI - 1;
+        for (;;) {
+            pthread_mutex_lock(&B->mu);
+            const int64_t k = B->next++;
+            pthread_mutex_unlock(&B->mu);
+            if (ilo + k > ihi) break;
+            blocked_tile(B, ilo + k, d - (ilo + k), row, &best);
+        }
+        /* Everyone has finished diagonal d; thread 0 resets the counter
+         * between the two barriers. */
+        pthread_barrier_wait(&B->bar);
+        if (arg->id == 0) B->next = 0;
+        pthread_barrier_wait(&B->bar);
+    }
+    B->best[arg->id] = best;
+    free(row);
+    return NULL;
+}
+
+int32_t oracle_blocked(int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2, int textbook,
+                       int threads, int64_t tile, int32_t *max_out)
+{
+    if (max_out) *max_out = 0;
+    if (n1 <= 0 || n2 <= 0) {
+        if (problem != ORACLE_ED) return 0;
+        return n2 == 0 ? ed_top(n1, textbook) : (int32_t)n2;
+    }
+    if (threads < 1) threads = 1;
+    if (tile < 16) tile = 16;
+    Blocked B;
+    memset(&B, 0, sizeof(B));
+    B.problem = problem;
+    B.textbook = textbook;
+    B.threads = threads;
+    B.a = a;
+    B.b = b;
+    B.n1 = n1;
+    B.n2 = n2;
+    B.tile = tile;
+    B.nI = (n2 + tile - 1) / tile;
+    B.nJ = (n1 + tile - 1) / tile;
+    B.top = (int32_t *)malloc(((size_t)n1 + 1) * sizeof(int32_t));
+    B.left = (int32_t *)malloc(((size_t)n2 + 1) * sizeof(int32_t));
+    B.corner = (int32_t *)malloc((size_t)(B.nI + 1) * (size_t)(B.nJ + 1) * sizeof(int32_t));
+    B.best = (int32_t *)calloc((size_t)threads, sizeof(int32_t));
+    BlockedArg *args = (BlockedArg *)malloc((size_t)threads * sizeof(BlockedArg));
+    pthread_t *tid = (pthread_t *)malloc((size_t)threads * sizeof(pthread_t));
+    if (!B.top || !B.left || !B.corner || !B.best || !args || !tid) {
+        free(B.top); free(B.left); free(B.corner); free(B.best); free(args); free(tid);
+        return -1;
+    }
+    for (int64_t x = 0; x <= n1; ++x) B.top[x] = (problem == ORACLE_ED) ? ed_top(x, textbook) : 0;
+    for (int64_t y = 0; y <= n2; ++y) B.left[y] = (problem == ORACLE_ED) ? (int32_t)y : 0;
+    for (int64_t I = 0; I <= B.nI; ++I)
+        for (int64_t J = 0; J <= B.nJ; ++J) {
+            int32_t v = 0;
+            if (problem == ORACLE_ED) {
+                const int64_t y = I * tile < n2 ? I * tile : n2, x = J * tile < n1 ? J * tile : n1;
+                v = (I == 0) ? ed_top(x, textbook) : (J == 0 ? (int32_t)y : 0);
+            }
+            B.corner[I * (B.nJ + 1) + J] = v;
+        }
+    pthread_barrier_init(&B.bar, NULL, (unsigned)threads);
+    pthread_mutex_init(&B.mu, NULL);
+    for (int t = 0; t < threads; ++t) {
+        args[t].B = &B;
+        args[t].id = t;
+        pthread_create(&tid[t], NULL, blocked_worker, &args[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+    pthread_barrier_destroy(&B.bar);
+    pthread_mutex_destroy(&B.mu);
+    int32_t m = 0;
+    for (int t = 0; t < threads; ++t) m = imax(m, B.best[t]);
+    if (max_out && problem == ORACLE_SW) *max_out = m;
+    const int32_t r = B.top[n1];
+    free(B.top); free(B.left); free(B.corner); free(B.best); free(args); free(tid);
+    return r;
+}

@@ -24,6 +24,11 @@ int oracle_last_row(int problem, const int32_t *a, int64_t n1,
  * 2D-SOR/nested_loop/GPU_Hyperlane_Share.cu): op 0 = SAT, 1 = SOR. */
 enum { ORACLE_SAT = 0, ORACLE_SOR = 1 };
 void oracle_stencil(int op, int64_t n1, int64_t n2, int32_t *t, int64_t ld);
+/* The same results computed by `threads` threads over anti-diagonals of
+ * tile x tile blocks (the all-cores CPU baseline). */
+int32_t oracle_blocked(int problem, const int32_t *a, int64_t n1,
+                       const int32_t *b, int64_t n2, int textbook,
+                       int threads, int64_t tile, int32_t *max_out);
 void oracle_batch(const int32_t *problems, const int32_t *const *as,
                   const int64_t *n1s, const int32_t *const *bs,
                   const int64_t *n2s, int64_t npairs, int32_t *results,
