@@ -1,30 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the wavefront DP hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ed|sw|lcs|batch]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sw|ed|lcs|batch]
 
-Default workload (the configuration the headline metric is quoted on that
-fits one GPU): edit distance (ED.cpp semantics) between two i.i.d. uniform
-20-letter sequences of 65536 symbols, seed 2 -- 4.29e9 DP cells.  A step is
-one pass of the DP over the pair with its codes resident in HBM (one
-wf_plan_run: reset + one persistent launch, synchronous).
+Default workload (the configuration the headline metric, % of int32 VALU
+peak, is quoted on): Smith-Waterman (SW.cpp scoring +3/-3/-2) of two
+131072-long 4-letter sequences, seed 3, with a planted 4096-long homologous
+segment (10% substitutions) -- 1.72e10 DP cells.  A step is one pass of the
+DP over the pair with its codes resident in HBM (one wf_plan_run: reset +
+one persistent launch, synchronous).
 
 One JSON line on stdout:
   value        GCUPS over the K timed steps (wall clock, whole job)
   roofline     the engine kernel against the int32 VALU peak of the chip
                (256 CUs x 64 lanes x max engine clock): achieved = algorithmic
-               int ops per launch (ED 7, SW 8, LCS 4 per cell) / mean launch
-               time from HIP events on the library's stream
-  cpu_baseline the oracle (C restatement of ED.cpp, one thread) timed on this
-               host on the same pair
-  extra        the other headline measurements: Smith-Waterman 131072^2
-               single pair; the 4096-pair batch against the loop of one
-               launch per pair; the reference's naive baselines (one launch
-               per cell or tile anti-diagonal) on LCS 8192^2; the served
-               Poisson stream (p50/p99 latency and GCUPS per rate, continuous
-               batching against one launch per request); the SAT and 2D-SOR
-               table sweeps at 2^15 x 2^15 (the reference's nvprof runs of
-               those programs) against the HBM bandwidth they are bound by
+               int ops per launch (SW 8, ED 7, LCS 4 per cell) / mean launch
+               time from HIP events on the library's stream; traffic = HBM
+               bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+               (profiles/traffic.json), when measured on this kernel variant
+  cpu_baseline the oracle (C restatement of SW.cpp) timed on this host on the
+               same pair: one thread over the whole pair, plus the all-cores
+               anti-diagonal tile variant (thread count stated)
+  extra        the other measurements: edit distance 65536^2 and LCS 8192^2
+               single pairs; the 4096-pair SW batch (one persistent launch,
+               eager and replayed from a hipGraph, tile from the shipped
+               autotune table) against the loop of one launch per pair; the
+               reference's naive baselines (one launch per cell or tile
+               anti-diagonal) on LCS 8192^2; the served Poisson stream (p50/p99
+               latency and GCUPS per rate, continuous batching against one
+               launch per request); the SAT and 2D-SOR table sweeps at
+               2^15 x 2^15; the reference's published workload (LCS of its
+               own 2^15 input file) through the drop-in LCS() call
 """
 from __future__ import annotations
 
@@ -128,33 +134,43 @@ def stencil_measure(ctx, exp: int = 15, steps: int = 3, warmup: int = 1, seed: i
     return out
 
 
-# The reference's only published timing of this path: LCS 32768 x 32768
-# (4 letters) through the hyperlane Share host function, 13.6486 s for 100
-# calls on a GeForce GTX 1080 Ti, every call including the table allocation,
-# the copies and the kernels (LCS/GPU_Hyperlane_Share.txt:103).
+# The reference's only published timing of this path: LCS of its own
+# 32768 x 32768 input file (LCS/Data/str1_2_15_str2_2_15.txt, committed as
+# tests/golden/data/) through the hyperlane Share host function, 13.6486 s
+# for 100 calls on a GeForce GTX 1080 Ti, every call including the table
+# allocation, the copies and the kernels (LCS/GPU_Hyperlane_Share.txt:103),
+# printing lcs length 21469 (:102).
 PUBLISHED_LCS_32768_MS_PER_CALL = 136.486
+PUBLISHED_LCS_32768_LENGTH = 21469
 
 
-def lcs_published_measure(ctx, calls: int = 20, seed: int = 7) -> dict:
-    """The same workload through the drop-in path (host arrays in, score
-    out: symbol encoding, upload, launch and result copy per call), timed
-    like the reference's driver: wall clock of whole calls."""
-    from gpumemefficient_amd import datagen as G
-    a, b = G.uniform_pair(32768, 32768, 4, seed)
-    ctx.align("LCS", a, b)
+def lcs_published_measure(calls: int = 20) -> dict:
+    """The same input through the drop-in call the reference's driver makes,
+    LCS(n1, n2, arr1, arr2, paddX, paddY, table) of libwfdp_compat_share.so
+    (host arrays in, score out: encoding, upload, launch and result copy in
+    every call), timed like the driver: wall clock of whole calls."""
+    import ctypes as C
+    from gpumemefficient_amd.datafile import read_input
+    a, b = read_input(ROOT / "tests" / "golden" / "data" / "str1_2_15_str2_2_15.txt")
+    lib = C.CDLL(str(ROOT / "gpumemefficient_amd" / "lib" / "libwfdp_compat_share.so"))
+    fn = getattr(lib, "_Z3LCSiiPiS_iiS_")  # int LCS(int, int, int*, int*, int, int, int*)
+    I = C.POINTER(C.c_int)
+    fn.argtypes = [C.c_int, C.c_int, I, I, C.c_int, C.c_int, I]
+    fn.restype = C.c_int
+    pa, pb = a.ctypes.data_as(I), b.ctypes.data_as(I)
+    length = fn(a.size, b.size, pa, pb, 3, 1, None)
+    assert length == PUBLISHED_LCS_32768_LENGTH, length
     t0 = time.perf_counter()
-    kms = []
     for _ in range(calls):
-        r = ctx.align("LCS", a, b)
-        kms.append(r.kernel_ms)
+        length = fn(a.size, b.size, pa, pb, 3, 1, None)
     dt = time.perf_counter() - t0
     ms = dt * 1e3 / calls
-    cells = 32768 * 32768
-    return {"workload": f"lcs_32768x32768_alpha4_seed{seed}", "calls": calls,
+    cells = a.size * b.size
+    return {"workload": "lcs_32768x32768_reference_input_LCS_Data_str1_2_15_str2_2_15", "calls": calls,
+            "entry": "LCS(n1, n2, arr1, arr2, paddX, paddY, table) of libwfdp_compat_share.so",
             "ms_per_call_end_to_end": round(ms, 3), "gcups_end_to_end": round(cells / ms / 1e6, 1),
-            "kernel_ms": round(sum(kms) / len(kms), 3),
             "published_ms_per_call_gtx1080ti": PUBLISHED_LCS_32768_MS_PER_CALL,
-            "speedup_vs_published": round(PUBLISHED_LCS_32768_MS_PER_CALL / ms, 1), "lcs_length": r.last}
+            "speedup_vs_published": round(PUBLISHED_LCS_32768_MS_PER_CALL / ms, 1), "lcs_length": length}
 
 
 def time_plan(plan, steps: int, warmup: int, per_pair: bool = False):
@@ -168,10 +184,33 @@ def time_plan(plan, steps: int, warmup: int, per_pair: bool = False):
     return dt, kms
 
 
+def _host_cpu() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_threads() -> int:
+    """Host threads this job may use: the box's share (OMP_NUM_THREADS when
+    set), at most 16."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, min(16, n))
+
+
 def cpu_baseline(jobs, prob: str, budget_s: float = 20.0, min_s: float = 10.0):
-    """Time the oracle (single thread) on the pair: on its first rows (a band
-    of the same table) if the whole pair would take longer than the budget,
-    else on the whole pair, repeated until at least `min_s` of CPU work."""
+    """Time the oracle on the pair: one thread (like the reference programs)
+    on its first rows (a band of the same table) if the whole pair would take
+    longer than the budget, else on the whole pair, repeated until at least
+    `min_s` of CPU work; then the all-cores variant (anti-diagonals of
+    256 x 256 tiles on every thread this job may use) on the whole pair,
+    repeated to at least `min_s` / 2."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     a, b = jobs[0].a, jobs[0].b
@@ -191,16 +230,72 @@ def cpu_baseline(jobs, prob: str, budget_s: float = 20.0, min_s: float = 10.0):
     what = "the whole" if rows == b.size else f"rows 0..{rows - 1} of the"
     sample = (f"{prob} oracle (C restatement, -O3, 1 thread) on {what} {b.size}x{a.size} pair, "
               f"{reps} run(s): {cells:.3e} cells in {dt:.1f} s")
-    host = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                host = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    threads = _cpu_threads()
+    want = O.score(prob, a, b) if rows == b.size else None
+    preps, pdt = 0, 0.0
+    while preps == 0 or pdt < min_s / 2:
+        t0 = time.perf_counter()
+        got = O.blocked(prob, a, b, threads, 256)
+        pdt += time.perf_counter() - t0
+        preps += 1
+    if want is not None:
+        assert got == want, (got, want)
+    pcells = a.size * b.size * preps
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port", "sample": sample,
-            "host_cpu": host, "host_threads": os.cpu_count()}
+            "host_cpu": _host_cpu(), "host_threads": os.cpu_count(),
+            "all_cores": {"value": round(pcells / pdt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+                          "sample": (f"{prob} oracle over anti-diagonals of 256x256 tiles, {threads} threads, on the "
+                                     f"whole {b.size}x{a.size} pair, {preps} run(s): {pcells:.3e} cells in "
+                                     f"{pdt:.1f} s")}}
+
+
+def single_pair_extra(ctx, name: str, peak: float, steps: int = 3) -> dict:
+    """One of the other single-pair workloads: kernel time, GCUPS and the
+    fraction of the int32 VALU peak (algorithmic ops)."""
+    jobs, prob, config = make_workload(name)
+    p = ctx.plan(jobs)
+    _, kms = time_plan(p, steps, 1)
+    r = p.results()[0]
+    k = sum(kms) / len(kms)
+    cells = int(jobs[0].a.size) * int(jobs[0].b.size)
+    out = {"workload": config["workload"], "gcups": round(cells / k / 1e6, 1), "kernel_ms": round(k, 3),
+           "int_valu_peak_frac": round(OPS_PER_CELL[prob] * cells / (k * 1e-3) / 1e12 / peak, 4),
+           "ops_per_cell": OPS_PER_CELL[prob], "last": r.last, "engine": p.describe()}
+    if prob == "SW":
+        out["max_score"] = r.max_score
+    p.close()
+    return out
+
+
+def batch_extra(ctx) -> dict:
+    """The 4096-pair SW batch: one persistent launch (eager, then replayed
+    from a captured hipGraph) against the loop of one launch per pair, on the
+    same plan; the planner's tile comes from the shipped autotune table."""
+    bj, _, bc = make_workload("batch")
+    p = ctx.plan(bj)
+    _, bkms = time_plan(p, 5, 1)
+    res_eager = [(r.last, r.max_score) for r in p.results()]
+    p.capture()
+    gdt, gkms = time_plan(p, 5, 1)
+    res_graph = [(r.last, r.max_score) for r in p.results()]
+    assert res_graph == res_eager
+    _, pkms = time_plan(p, 3, 1, per_pair=True)
+    assert [(r.last, r.max_score) for r in p.results()] == res_eager
+    bcells = sum(int(j.a.size) * int(j.b.size) for j in bj)
+    bk, gk, pk = sum(bkms) / len(bkms), sum(gkms) / len(gkms), sum(pkms) / len(pkms)
+    out = {"workload": bc["workload"], "cells": bcells,
+           "persistent_gcups": round(bcells / bk / 1e6, 1), "persistent_ms": round(bk, 3),
+           "graph_replay_gcups": round(bcells / gk / 1e6, 1), "graph_replay_ms": round(gk, 3),
+           "per_pair_launch_gcups": round(bcells / pk / 1e6, 1), "per_pair_launch_ms": round(pk, 3),
+           "speedup": round(pk / min(bk, gk), 2), "engine": p.describe(), "tile_source": tile_source(ctx, bj)}
+    p.close()
+    return out
+
+
+def tile_source(ctx, jobs) -> str:
+    from gpumemefficient_amd import api
+    t = api.tuned(jobs, ctx)
+    return f"autotune table ({t.engine or 'default'} {t.strip_rows} rows, K={t.tile_w})" if t else "built-in defaults"
 
 
 def main() -> int:
@@ -208,9 +303,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ed", choices=["ed", "sw", "lcs", "batch", "stencil"])
+    ap.add_argument("--workload", default="sw", choices=["sw", "ed", "lcs", "batch", "stencil"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the SW single-pair, batch and serving measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other single pairs, batch and serving")
     ap.add_argument("--serve-requests", type=int, default=600, help="requests per rate in the serving measurement")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -278,30 +373,11 @@ def main() -> int:
 
     if not args.no_extra:
         extra = {}
-        if args.workload != "sw":
-            sj, sp, sc = make_workload("sw")
-            p = ctx.plan(sj)
-            sdt, skms = time_plan(p, 3, 1)
-            sr = p.results()[0]
-            sk = sum(skms) / len(skms)
-            scells = int(sj[0].a.size) * int(sj[0].b.size)
-            extra["sw_131072"] = {
-                "gcups": round(scells / sk / 1e6, 1), "kernel_ms": round(sk, 3),
-                "int_valu_peak_frac_8ops": round(8 * scells / (sk * 1e-3) / 1e12 / peak, 4),
-                "last": sr.last, "max_score": sr.max_score, "engine": p.describe()}
-            p.close()
+        for name, key in (("sw", "sw_131072"), ("ed", "ed_65536"), ("lcs", "lcs_8192")):
+            if args.workload != name:
+                extra[key] = single_pair_extra(ctx, name, peak)
         if args.workload != "batch":
-            bj, bp, bc = make_workload("batch")
-            p = ctx.plan(bj)
-            bdt, bkms = time_plan(p, 5, 1)
-            pdt, pkms = time_plan(p, 3, 1, per_pair=True)
-            bcells = sum(int(j.a.size) * int(j.b.size) for j in bj)
-            bk, pk = sum(bkms) / len(bkms), sum(pkms) / len(pkms)
-            extra["batch_4096"] = {
-                "cells": bcells, "persistent_gcups": round(bcells / bk / 1e6, 1), "persistent_ms": round(bk, 3),
-                "per_pair_launch_gcups": round(bcells / pk / 1e6, 1), "per_pair_launch_ms": round(pk, 3),
-                "speedup": round(pk / bk, 2), "engine": p.describe()}
-            p.close()
+            extra["batch_4096"] = batch_extra(ctx)
         # The reference's naive GPU baselines on the LCS 8192 x 8192 pair:
         # one launch per cell anti-diagonal, one launch per tile
         # anti-diagonal, against the strip engine on the same pair.
@@ -334,7 +410,7 @@ def main() -> int:
         serve["saturated_speedup"] = round(sat_c / sat_p, 2) if sat_p else None
         extra["serving"] = serve
         extra["stencil_32768"] = stencil_measure(ctx)
-        extra["lcs_32768_drop_in"] = lcs_published_measure(ctx)
+        extra["lcs_32768_reference_input_drop_in"] = lcs_published_measure()
         line["extra"] = extra
 
     if not args.no_cpu:

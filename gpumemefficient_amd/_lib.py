@@ -33,6 +33,7 @@ WF_TILE_ASYNC = 0x8
 WF_TILE_NO_COPYBACK = 0x10
 WF_TILE_BITPAR = 0x20
 WF_TILE_BITPAR_STAGED = 0x40
+WF_TILE_HINT = 0x80
 ENGINES = ("lockstep", "async", "bitpar", "bitpar_staged")
 
 WF_TABLE_MAX_CELLS = 1 << 28
@@ -53,7 +54,7 @@ EXPORTS = [
     "wf_plan_results", "wf_plan_kernel_ms", "wf_plan_describe", "wf_plan_destroy",
     "wf_variant_count", "wf_variant_info", "wf_plan_capture", "wf_autotune", "wf_tune_load", "wf_tune_save",
     "wf_tune_clear", "wf_server_create", "wf_server_submit", "wf_server_poll", "wf_server_wait_idle",
-    "wf_server_stats", "wf_server_replay", "wf_server_destroy", "wf_align_noalloc",
+    "wf_server_stats", "wf_server_replay", "wf_server_destroy", "wf_align_noalloc", "wf_tune_lookup_tile",
 ]
 
 
@@ -141,6 +142,8 @@ def load() -> C.CDLL:
     lib.wf_tune_load.restype = C.c_int
     lib.wf_tune_save.argtypes = [C.c_char_p]
     lib.wf_tune_save.restype = C.c_int
+    lib.wf_tune_lookup_tile.argtypes = [P, C.POINTER(wf_job), C.c_int64, C.POINTER(wf_tile)]
+    lib.wf_tune_lookup_tile.restype = C.c_int
     lib.wf_tune_clear.argtypes = []
     lib.wf_tune_clear.restype = C.c_int
     lib.wf_plan_results.argtypes = [P, C.POINTER(wf_result), C.c_int64]

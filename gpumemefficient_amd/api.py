@@ -76,19 +76,22 @@ class Tile:
     ed_textbook: bool = False
     cache: bool = False
     engine: str = ""      # "lockstep", "async", "bitpar", "bitpar_staged" or "" (default)
+    hint: bool = False    # the fields are hints (WF_TILE_HINT): the planner picks the variant
 
     def _c(self) -> _lib.wf_tile:
         flags = ((_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
                  | (_lib.WF_TILE_LOCKSTEP if self.engine == "lockstep" else 0)
                  | (_lib.WF_TILE_ASYNC if self.engine == "async" else 0)
                  | (_lib.WF_TILE_BITPAR if self.engine == "bitpar" else 0)
-                 | (_lib.WF_TILE_BITPAR_STAGED if self.engine == "bitpar_staged" else 0))
+                 | (_lib.WF_TILE_BITPAR_STAGED if self.engine == "bitpar_staged" else 0)
+                 | (_lib.WF_TILE_HINT if self.hint else 0))
         return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
 
     @classmethod
     def _from(cls, t: _lib.wf_tile) -> "Tile":
         eng = ("lockstep" if t.flags & _lib.WF_TILE_LOCKSTEP else "async" if t.flags & _lib.WF_TILE_ASYNC
-               else "bitpar" if t.flags & _lib.WF_TILE_BITPAR else "")
+               else "bitpar" if t.flags & _lib.WF_TILE_BITPAR
+               else "bitpar_staged" if t.flags & _lib.WF_TILE_BITPAR_STAGED else "")
         return cls(t.tile_w, t.strip_rows, t.rows_per_lane, bool(t.flags & _lib.WF_TILE_ED_TEXTBOOK),
                    bool(t.flags & _lib.WF_TILE_CACHE), eng)
 
@@ -315,6 +318,19 @@ def tune_clear() -> None:
     _lib.load().wf_tune_clear()
 
 
+def tuned(jobs: Iterable[Job], ctx: "Context | None" = None) -> Tile | None:
+    """The tile the autotune table holds for `jobs` on this device (what a
+    plan without an explicit tile uses), or None."""
+    c = ctx or default_context()
+    ja = _JobArray(jobs)
+    t = _lib.wf_tile()
+    st = c._lib.wf_tune_lookup_tile(c._h, ja.c, ja.n, C.byref(t))
+    if st == _lib.WF_ERR_NOKERNEL:
+        return None
+    check(st, "wf_tune_lookup_tile")
+    return Tile._from(t)
+
+
 _default: Context | None = None
 
 
@@ -327,8 +343,9 @@ def default_context() -> Context:
 
 def _tile_from_ref(tX: int | None, tY: int | None, cache: bool = False) -> Tile:
     # tY rows per strip and tX columns per tile, as on the reference's argv
-    # (SmithWaterman/GPU_Hyperlane_Share.cpp:56-59).
-    return Tile(tile_w=int(tX or 0), strip_rows=int(tY or 0), cache=cache)
+    # (SmithWaterman/GPU_Hyperlane_Share.cpp:56-59), passed as hints: the
+    # planner runs its fastest variant (every variant gives the same score).
+    return Tile(tile_w=int(tX or 0), strip_rows=int(tY or 0), cache=cache, hint=True)
 
 
 def LCS(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
@@ -379,4 +396,4 @@ def SOR(n1: int, n2: int, table, tX: int | None = None, tY: int | None = None, k
 
 
 __all__ = ["Context", "Plan", "Job", "Tile", "Result", "WfError", "LCS", "SW", "ED", "SOR", "default_context",
-           "variants", "tile_for"]
+           "variants", "tile_for", "tuned"]

@@ -180,6 +180,13 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
         E = WF_ENGINE_LOCKSTEP;
         R = W = K = 0;
     }
+    // A hint tile (the reference's tX, tY) only keeps its flags: the
+    // variant is the planner's own choice.
+    wf_tile hinted{};
+    if (tile && (tile->flags & WF_TILE_HINT)) {
+        hinted.flags = tile->flags & ~WF_TILE_HINT;
+        tile = &hinted;
+    }
     const bool cache = tile && (tile->flags & WF_TILE_CACHE);
     const bool explicit_tile = tile && (tile->tile_w > 0 || tile->strip_rows > 0 || tile->rows_per_lane > 0);
     if (!explicit_tile && !cache && ctx) {
@@ -767,6 +774,7 @@ int wf_ctx_create(int device, wf_ctx **out)
     // Set up this device's static scratch area of wf_align_noalloc (once per
     // device); a failure here is reported by that entry point's own call.
     (void)wf_noalloc_setup(device, c->stream, nullptr);
+    wf_tune_autoload();
     *out = c;
     return WF_OK;
 }

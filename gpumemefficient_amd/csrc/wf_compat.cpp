@@ -124,10 +124,14 @@ int ED(int n1, int n2, int *arr1, int *arr2)
 
 #else  // hyperlane drivers
 
+// The reference sizes tX x tY tiles for its shared-memory staging on a
+// Pascal GPU; here they are hints (WF_TILE_HINT): every variant gives the
+// same score, and the planner runs its fastest one for the pair.  The Cache
+// library keeps its policy (strip boundaries through L2 only).
 #if defined(WF_COMPAT_POLICY_CACHE)
 static const int kDefaultTX = 256, kDefaultTY = 1024, kPolicyFlags = WF_TILE_CACHE;
 #else
-static const int kDefaultTX = 32, kDefaultTY = 256, kPolicyFlags = 0;
+static const int kDefaultTX = 32, kDefaultTY = 256, kPolicyFlags = WF_TILE_HINT;
 #endif
 
 // With WFDP_DEBUG_TABLE=1 a non-null table of a square pair receives the

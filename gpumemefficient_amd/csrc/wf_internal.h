@@ -99,6 +99,8 @@ int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, 
 const WfVariant *wf_pick_variant(struct wf_ctx *ctx, const wf_tile *tile, const wf_job *jobs, int64_t njobs,
                                  int64_t max_rows, int64_t total_cells);
 void wf_tune_key(const wf_job *jobs, int64_t njobs, int *problem, int *batch, int *bucket);
+// Load the autotune table shipped with the library (once per process).
+void wf_tune_autoload();
 bool wf_tune_lookup(const char *arch, int cus, int problem, int batch, int bucket, int *engine, int *R, int *W,
                     int *K);
 int wf_default_waves(int64_t max_rows, int64_t total_cells, int npairs);

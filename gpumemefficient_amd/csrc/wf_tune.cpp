@@ -9,11 +9,13 @@
 // The table is a small text file (one entry per line, '#' comments):
 //   <arch> <cus> <problem> <class> <bucket> <engine> <rows_per_lane> <waves> <chunk> <ms>
 // wf_pick_variant() consults it before the built-in defaults.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -99,6 +101,28 @@ static void tune_insert(const TuneEntry &n)
     g_table.push_back(n);
 }
 
+// Load the shipped table once per process: $WFDP_TUNE, else
+// <directory of libwfdp.so>/../tune/wfdp_tune.txt.
+void wf_tune_autoload()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *env = std::getenv("WFDP_TUNE");
+        if (env && std::strcmp(env, "0") == 0) return;
+        std::string path;
+        if (env && *env) {
+            path = env;
+        } else {
+            Dl_info info{};
+            if (!dladdr(reinterpret_cast<const void *>(&wf_tune_autoload), &info) || !info.dli_fname) return;
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/../tune/wfdp_tune.txt";
+        }
+        (void)wf_tune_load(path.c_str());
+    });
+}
+
 extern "C" {
 
 int wf_tune_load(const char *path)
@@ -152,6 +176,17 @@ static int32_t engine_flag(int engine)
            : engine == WF_ENGINE_ASYNC  ? WF_TILE_ASYNC
            : engine == WF_ENGINE_BITPAR ? WF_TILE_BITPAR
                                         : WF_TILE_BITPAR_STAGED;
+}
+
+int wf_tune_lookup_tile(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, wf_tile *out)
+{
+    if (!ctx || !jobs || njobs <= 0 || !out) return WF_ERR_INVALID;
+    int p, b, bk, e, R, W, K;
+    wf_tune_key(jobs, njobs, &p, &b, &bk);
+    if (!wf_tune_lookup(ctx->prop.gcnArchName, ctx->prop.multiProcessorCount, p, b, bk, &e, &R, &W, &K))
+        return WF_ERR_NOKERNEL;
+    *out = wf_tile{K, 64 * W * R, R, engine_flag(e)};
+    return WF_OK;
 }
 
 int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_tile *best, double *best_ms)

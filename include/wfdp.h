@@ -88,6 +88,11 @@ typedef struct {
                                       strip of the launch gets a CU of its own */
 #define WF_TILE_NO_COPYBACK 0x10 /* wf_stencil: leave the host table unchanged (the
                                     reference never copies its result back) */
+#define WF_TILE_HINT 0x80       /* tile_w, strip_rows and rows_per_lane are hints from a
+                                   caller that sizes tiles for another machine (the
+                                   reference's tX, tY): the planner picks its own variant
+                                   (autotune table, then defaults); every variant gives
+                                   the same results */
 
 typedef struct {
     int32_t last;      /* bottom-right cell: the score the reference prints */
@@ -245,6 +250,12 @@ int wf_autotune(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, int reps, wf_til
 int wf_tune_load(const char *path);
 int wf_tune_save(const char *path);
 int wf_tune_clear(void);
+/* The tile the autotune table holds for `jobs` on this context's device
+ * (what a plan without an explicit tile would use), or WF_ERR_NOKERNEL if
+ * it holds none.  The first context of a process loads the table shipped
+ * with the library (tune/wfdp_tune.txt beside lib/), or the file named by
+ * WFDP_TUNE in the environment; WFDP_TUNE=0 loads none. */
+int wf_tune_lookup_tile(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, wf_tile *out);
 
 /* Compiled kernel variants (engine 0 = lock-step, 1 = asynchronous, 2 =
  * bit-parallel, whose rows_per_lane is the 32 bits of a lane's word). */

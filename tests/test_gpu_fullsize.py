@@ -44,6 +44,29 @@ def test_lcs_8192(ctx, tile):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tx", [128, 256])
+def test_lcs_8192_through_the_drop_in_driver(tmp_path, tx):
+    """LCS of the 8192 x 8192 seed-1 pair through the drop-in
+    LCS/GPU_Hyperlane_Share at tile widths 128 and 256 with 1024 threads per
+    tile (the driver's 5-argument form, e1 e2 tX tY itr; the reference's own
+    host hard-codes 32 x 256, LCS/GPU_Hyperlane_Share.cu:703-704): the
+    printed length is the oracle's."""
+    import re
+    import subprocess
+    from gpumemefficient_amd.datafile import write_input
+    exe = Path(__file__).resolve().parent.parent / "gpumemefficient_amd" / "bin" / "LCS" / "GPU_Hyperlane_Share"
+    a, b = G.lcs_pair()
+    write_input(tmp_path / "Data" / "str1_2_13_str2_2_13.txt", a, b)
+    r = subprocess.run([str(exe), "13", "13", str(tx), "1024", "1"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == "input data loaded"
+    assert lines[1] == f"lcs length: {FIX['lcs_8192']['last']}"
+    assert re.fullmatch(r"execution time: [0-9.e+-]+ ms\.", lines[2]), lines[2]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tile", [None, LOCKSTEP], ids=["default", "lockstep"])
 def test_ed_65536(ctx, tile):
     a, b = G.ed_pair()
