@@ -247,19 +247,18 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
     // strips than CUs is bound by throughput, where the lock-step engine's
     // higher occupancy wins (measured on MI355X: SW 131072^2 and ED 65536^2
     // against the 4096-pair batch).
-    // The async engine's counter group G trades per-step cost (one check per
-    // group) against the lag a wave adds to the chain (about 2G + 63 steps):
-    // with half a chip of strips or fewer the steps dominate (ED 65536^2:
-    // G = 32 3.47 ms, G = 16 3.57 ms), with more the lag does (SW 131072^2:
-    // G = 16 8.29 ms, G = 32 8.41 ms).
-    int async_k = 16;
+    // The async engine's counter group G trades per-step cost (a check, a
+    // branch that ends the compiler's scheduling region, per group) against
+    // the lag a wave adds to the chain (64 + G steps): G = 32 measured best
+    // on both single pairs (SW 131072^2: G = 8 / 16 / 32 8.43 / 7.88 /
+    // 7.80 ms; ED 65536^2 on this engine: 3.77 / 3.39 / 3.26 ms).
+    const int async_k = 32;
     {
         int64_t strips = 0;
         for (int64_t i = 0; i < njobs; ++i)
             if (jobs[i].n1 > 0 && jobs[i].n2 > 0) strips += (jobs[i].n2 + 511) / 512;
         const int cus = ctx ? ctx->prop.multiProcessorCount : 256;
         if (strips <= 2LL * cus) E = WF_ENGINE_ASYNC;
-        if (strips <= cus / 2) async_k = 32;
     }
     if (tile && (tile->flags & WF_TILE_ASYNC)) E = WF_ENGINE_ASYNC;
     if (tile && (tile->flags & WF_TILE_LOCKSTEP)) E = WF_ENGINE_LOCKSTEP;

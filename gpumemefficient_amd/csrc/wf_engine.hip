@@ -8,10 +8,10 @@
 //   * W compute waves stack vertically (wave w owns rows w*64*R ... of the
 //     strip).  Each advances in groups of G steps and publishes its progress
 //     in an LDS counter; the wave below starts a group once the counter says
-//     the lane-63 values it needs are in the LDS ring.  The check uses the
-//     counter read one group earlier, so the read latency overlaps compute,
-//     and the steady-state lag between two waves is 64 + 2G steps instead of
-//     the two chunks of the lock-step engine.
+//     the lane-63 values it needs are in the LDS ring.  The counter is read
+//     just before the group's first operands and checked when the group
+//     starts (check on use), so the steady-state lag between two waves is
+//     64 + G steps plus the LDS latency.
 //   * One extra I/O wave per workgroup owns all global-memory traffic.  It
 //     stages the strip's top boundary (the previous strip's bottom row) and
 //     the column codes into LDS as soon as the producing workgroup has
@@ -105,6 +105,7 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 #ifndef WFK_IN_SLEEP
 #define WFK_IN_SLEEP 16
 #endif
+
 #ifndef WFK_LDS_HW_FENCE
 #define WFK_LDS_HW_FENCE 0
 #endif
@@ -155,16 +156,18 @@ __device__ __forceinline__ void load_block(Block &b, const int *ring_prev, const
 
 // U steps of one compute wave, starting at step q0*G (U a multiple of G, G
 // of kB; the progress counters work in groups of G steps).  `blk` holds the
-// first block on entry and the block after the last on return.  At the last
-// block of each group the counters are checked for the next group: its
-// inputs must be there (producer) and its output slots free (consumer).
-// The counters checked were loaded half a block earlier, in the previous
-// group's last block (`vin`/`vout`), so the check does not wait on a load,
-// and the reads after it see everything the counters vouch for (LDS
-// executes one wave's accesses in order).  A short counter sends the
-// wave to `wait_next`, out of line.  The group's progress is published after
-// its last block.  There is one load site per block, so the loop-carried
-// block stays in the same VGPRs.  Returns false if a wait was abandoned.
+// first block's operands on entry and the block after the last on return.
+// Check on use: the counters that vouch for group g are read just before the
+// operands of g's first block (LDS executes one wave's accesses in order, so
+// operands read after a counter that shows the producer's stores see those
+// stores), and checked when the group starts.  A short counter sends the
+// wave to `wait_next` (out of line), after which it re-reads the block's
+// operands.  A wave thus trails its producer by 64 + G steps plus the LDS
+// latency, instead of 64 + 2G with a check one group ahead.  The group's
+// progress is published after its last block (publishing after every block
+// measured slower: the step is bound by the wave's instruction issue).  There is one load site per
+// block on the fast path, so the loop-carried block stays in the same VGPRs.
+// Returns false if a wait was abandoned.
 template <int P, bool CMP, int R, int G, int U, int AW, int RS, bool MASKED, bool TRACK, class NeedIn,
           class NeedOut, class WaitFn, class PubFn>
 __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int lane, int n1, int xlo,
@@ -175,6 +178,7 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
                                           PubFn &&publish)
 {
     static_assert(U % G == 0 && G % kB == 0, "bad unroll");
+    (void)ngr;
     const int s0 = q0 * G;
     // s0 is a multiple of U, so neither the ring nor the (mirrored) code
     // window wraps inside the iteration; only the next iteration's first
@@ -184,8 +188,30 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
     int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
     int *cap_last = (lane == 63) ? ring_cur + ((s0 + U) & (RS - 1)) : dummy + lane + (U - 1);
     bool ok = true;
+    // The consumer of this wave's ring trails it by about 64 + G steps and
+    // the ring holds RS = 256 columns, so its counter (vout) is read and
+    // checked only once per kOutEvery groups, for all of them.
+    constexpr int kOutEvery = (G >= 32) ? 1 : 32 / G;
 #pragma unroll
     for (int q = 0; q < U; q += kB) {
+        const int gi = q0 + q / G;  // group of this block
+        if (q % G == 0) {
+            // Every lane holds its own copy of the counters, so the compare
+            // is wave-uniform: one VALU compare and a branch on its mask.
+            bool short_ = vin < need_in(gi);
+            // Global group index (the steady loop starts at multiples of
+            // kU / G groups, so there it is known at compile time).
+            const bool out_group = MASKED ? (gi % kOutEvery == 0) : ((q / G) % kOutEvery == 0);
+            if (out_group) short_ = short_ || vout < need_out(gi + kOutEvery - 1);
+            if (__builtin_expect(__builtin_amdgcn_ballot_w64(short_) != 0, 0)) {
+                ok = ok && wait_next(gi);
+#pragma unroll
+                for (int i = 0; i < kB / 4; ++i) blk.inj[i] = *reinterpret_cast<const int4 *>(ib + q + 4 * i);
+#pragma unroll
+                for (int u = 0; u < kB; ++u) blk.ca[u] = cp[q + u];
+            }
+            lds_acquire();
+        }
         int injv[kB];
         uint32_t cav[kB];
 #pragma unroll
@@ -193,13 +219,16 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
             injv[u] = blk.top(u);
             cav[u] = blk.ca[u];
         }
-        const int gi = q0 + q / G;  // group of this block
-        if (q % G == G - kB) {
-            // Wave-uniform: every lane reads its own copy of the counters.
-            const int slack = min(__builtin_amdgcn_readfirstlane(vin) - need_in(gi + 1),
-                                  __builtin_amdgcn_readfirstlane(vout) - need_out(gi + 1));
-            if (__builtin_expect(slack < 0 && gi + 1 < ngr, 0)) ok = ok && wait_next(gi + 1);
+        // The counters for the next group's check, read before that group's
+        // first operands (the scheduling barriers keep the compiler from
+        // moving either across the other).
+        if ((q + kB) % G == 0) {
+            // Only LDS accesses are kept from crossing (arithmetic may).
+            __builtin_amdgcn_sched_barrier(0x7);
+            vin = cnt_load(cin);
+            if (MASKED ? ((gi + 1) % kOutEvery == 0) : (((q + kB) / G) % kOutEvery == 0)) vout = cnt_load(cout);
             lds_acquire();
+            __builtin_amdgcn_sched_barrier(0x7);
         }
         // Past the last group the next block reads stale slots (in bounds)
         // that are never used.
@@ -219,23 +248,17 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
                 *cap_last = st.h[R - 1];
             else
                 cap[j] = st.h[R - 1];
-            // Counters for the next check, half a block before it (the
-            // scheduling barriers keep the compiler from sinking the loads
-            // to just before their use).
-            if (q % G == G - kB && u == WFK_CNT_AT) {
-                __builtin_amdgcn_sched_barrier(0);
-                vin = cnt_load(cin);
-                vout = cnt_load(cout);
-                __builtin_amdgcn_sched_barrier(0);
-            }
         }
         if (q % G == G - kB) publish(gi);
     }
     return ok;
 }
 
+// Forced inline into the kernel: left as a call it takes the kernel
+// arguments by address and copies them (and the pair) to scratch memory,
+// which the steady loop then reloads.
 template <int P, bool CMP, int R, int W, int G, int AW, int RS>
-__device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
+__device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
                             AsyncShared<W, G, AW, RS> &sh, int lane, int wave, uint32_t epoch, int task)
 {
 #ifdef WFK_TRACE
@@ -312,21 +335,23 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         const int ngr = (nsteps + G - 1) / G;
         // Group q reads input columns < (q+1)G: staged (w = 0) or written by
         // lane 63 of wave w-1, which finishes column c at its step c + 63.
-        auto need_in = [&](int q) { return w == 0 ? min(n1, (q + 1) * G) : min(nsteps, (q + 1) * G + 63); };
+        auto need_in = [&](int q) __attribute__((always_inline)) { return w == 0 ? min(n1, (q + 1) * G) : min(nsteps, (q + 1) * G + 63); };
         // Group q overwrites ring slots of columns up to (q+1)G - 64 - RS,
         // which the consumer must have read.
-        auto need_out = [&](int q) { return (q + 1) * G - 63 - RS; };
+        auto need_out = [&](int q) __attribute__((always_inline)) { return (q + 1) * G - 63 - RS; };
 
         // Wait until group q's inputs are there and its output slots free.
         // Every lane holds its own copy of a counter, so the values are read
         // through readfirstlane and the whole loop stays wave-uniform
-        // (scalar branches, no exec-mask bookkeeping per group).
-        auto ready = [&](int q) -> bool {
+        // (scalar branches, no exec-mask bookkeeping per group).  These
+        // helpers are forced inline: a helper left out of line takes its
+        // captures by address and moves the strip's state to scratch memory.
+        auto ready = [&](int q) __attribute__((always_inline)) -> bool {
             const int vi = __builtin_amdgcn_readfirstlane(cnt_load(cin));
             const int vo = __builtin_amdgcn_readfirstlane(cnt_load(cout));
             return vi >= need_in(q) && vo >= need_out(q);
         };
-        auto wait_ready = [&](int q) -> bool {
+        auto wait_ready = [&](int q) __attribute__((always_inline)) -> bool {
             if (!ready(q)) {
 #ifdef WFK_TRACE
                 const int64_t c0 = clock64();
@@ -358,27 +383,29 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         tr_nwait = 0;
         tr_wait = 0;
 #endif
-        load_block<AW, RS>(blk, sh.ring[w], sh.aw, 0, lane);
+        // Counters first, then the first block's operands (check on use).
         int vin = cnt_load(cin), vout = cnt_load(cout);
+        lds_acquire();
+        load_block<AW, RS>(blk, sh.ring[w], sh.aw, 0, lane);
         // Published progress, kept in a VGPR of its own (it is loaded, so the
         // compiler treats it as per-lane): materialising it from an SGPR right
         // before the store would reuse a register of a capture store still in
         // flight and make the wave wait for its block reads.
         int prog = cnt_load(cmine);  // 0: counters are cleared before each strip
-        auto publish = [&](int gi) {
+        auto publish = [&](int gi) __attribute__((always_inline)) {
             (void)gi;
             prog = min(nsteps, prog + G);
             lds_release();
             cnt_store(cmine, prog);
         };
-        auto wait_next = [&](int gi) { return wait_ready(gi); };
+        auto wait_next = [&](int gi) __attribute__((always_inline)) { return wait_ready(gi); };
         // Steps whose columns are valid for every lane run the unmasked body,
         // kU steps per loop iteration; the masked ones (lanes before column
         // 0 at the start, past column n1 - 1 at the end) run one group per
         // iteration.  One body per loop keeps `blk` in fixed registers.
         constexpr int UG = kU / G;
         const int xlo = free_head ? -64 : 0;
-        auto steady_from = [&](int q) {
+        auto steady_from = [&](int q) __attribute__((always_inline)) {
             return (free_head || q * G >= 63) && q * G + kU - 1 <= n1 - 1 && q + UG <= ngr;
         };
         int q = 0;
@@ -514,7 +541,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
         uint32_t c;
         int base;
     };
-    auto issue = [&](Poll &p) {
+    auto issue = [&](Poll &p) __attribute__((always_inline)) {
         p.base = in_next;
         const int x = in_next + lane;
         const bool in = x < n1;
@@ -526,7 +553,7 @@ __device__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pai
     // waves have freed: ring[0] slot of column c is free once wave 0 has
     // read it (steps < p0 done), its code slot once the last wave has
     // (it reads columns >= pl - 63).  Then poll again from in_next.
-    auto examine = [&](Poll &p, int p0, int pl) -> bool {
+    auto examine = [&](Poll &p, int p0, int pl) __attribute__((always_inline)) -> bool {
         const bool stale = (p.t >> 32) != (uint64_t)epoch;
         const uint64_t smask = __builtin_amdgcn_ballot_w64(stale);
         const int ready = smask ? (int)__builtin_ctzll(smask) : 64;
@@ -644,6 +671,7 @@ extern "C" const WfVariant wf_variants_async[] = {
     WF_ASYNC(1, 2, 16), WF_ASYNC(2, 2, 16), WF_ASYNC(4, 2, 16),
     WF_ASYNC(1, 1, 16), WF_ASYNC(2, 1, 16), WF_ASYNC(4, 1, 16), WF_ASYNC(8, 1, 16),
     WF_ASYNC(1, 8, 16), WF_ASYNC(2, 8, 16), WF_ASYNC(2, 3, 16), WF_ASYNC(2, 7, 16),
+    WF_ASYNC(1, 4, 32), WF_ASYNC(2, 2, 32), WF_ASYNC(1, 2, 32), WF_ASYNC(2, 1, 32), WF_ASYNC(1, 1, 32),
 };
 extern "C" const int wf_num_variants_async = sizeof(wf_variants_async) / sizeof(wf_variants_async[0]);
 
