@@ -9,7 +9,7 @@ SERVER_EXPORTS = ["wf_server_create", "wf_server_submit", "wf_server_poll", "wf_
 
 class wf_server_opts(C.Structure):
     _fields_ = [("streams", C.c_int32), ("max_batch", C.c_int32), ("hbm_budget", C.c_int64),
-                ("queue_capacity", C.c_int32), ("per_request", C.c_int32)]
+                ("queue_capacity", C.c_int32), ("per_request", C.c_int32), ("resident", C.c_int32)]
 
 
 class wf_completion(C.Structure):

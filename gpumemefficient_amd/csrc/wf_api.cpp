@@ -108,10 +108,32 @@ int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, 
     return WF_OK;
 }
 
+int wf_encode_job(const wf_job &j, int32_t tile_flags, bool perm4, uint8_t *a_out, uint8_t *b_out, int32_t *mode_out)
+{
+    int32_t mode = 0;
+    const int st = wf_encode_pair(j.problem, j.a, j.n1, j.b, j.n2, a_out, b_out, &mode);
+    if (st != WF_OK) return st;
+    if (j.problem == WF_ED && ((j.flags | tile_flags) & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
+    // SW with at most 4 symbols on a cell engine with two rows per lane:
+    // byte row tables and plain codes (WFK_MODE_PERM4).
+    if (perm4 && j.problem == WF_SW && !(mode & WFK_MODE_CMP)) {
+        bool small = true;
+        for (int64_t k = 0; small && k < j.n1; ++k) small = a_out[k] <= 12;
+        for (int64_t k = 0; small && k < j.n2; ++k) small = b_out[k] <= 12;
+        if (small) {
+            for (int64_t k = 0; k < j.n1; ++k) a_out[k] >>= 2;
+            for (int64_t k = 0; k < j.n2; ++k) b_out[k] >>= 2;
+            mode = (mode & WFK_MODE_TEXTBOOK) | WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD;
+        }
+    }
+    *mode_out = mode;
+    return WF_OK;
+}
+
 // Scores of degenerate pairs (an empty sequence), per the reference
 // boundaries: LCS/SW 0; ED D(n2,0) = n2, D(0,n1) = n1-1 (n1 >= 1, ED.cpp
 // :48-54) or n1 (textbook).
-static int32_t empty_score(int problem, int64_t n1, int64_t n2, int flags)
+int32_t wf_empty_score(int problem, int64_t n1, int64_t n2, int flags)
 {
     if (problem != WF_ED) return 0;
     if (n2 == 0) {
@@ -119,6 +141,12 @@ static int32_t empty_score(int problem, int64_t n1, int64_t n2, int flags)
         return (flags & WF_TILE_ED_TEXTBOOK) ? (int32_t)n1 : (int32_t)(n1 - 1);
     }
     return (int32_t)n2;
+}
+
+bool wf_perm4_enabled()
+{
+    const char *e = std::getenv("WFDP_PERM");
+    return !(e && e[0] == '0');
 }
 
 // ----------------------------------------------------------------------------
@@ -336,12 +364,7 @@ static bool split_enabled()
     return !(e && e[0] == '0');
 }
 // Byte row tables for SW pairs of at most 4 symbols (WFK_MODE_PERM4): on
-// unless WFDP_PERM=0.
-static bool perm4_enabled()
-{
-    const char *e = std::getenv("WFDP_PERM");
-    return !(e && e[0] == '0');
-}
+// unless WFDP_PERM=0 (wf_perm4_enabled).
 // Pairs narrower than this are not split.
 constexpr int64_t kSplitMinCols = 1024;
 
@@ -363,7 +386,7 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     const int64_t strip = 64LL * v->waves * v->rows_per_lane;
     const bool bitpar = v->engine == WF_ENGINE_BITPAR || v->engine == WF_ENGINE_BITPAR_STAGED;
     const bool perm4 = (v->engine == WF_ENGINE_ASYNC || v->engine == WF_ENGINE_LOCKSTEP) && v->rows_per_lane == 2 &&
-                       perm4_enabled();
+                       wf_perm4_enabled();
 
     // Bit-parallel pairs are split into two column halves computed at once
     // (WfSplitDev) when the launch has room for twice its strips: a single
@@ -421,7 +444,7 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
             p.b_off = code_bytes;
             code_bytes += (j.n2 + 15) & ~15LL;
             ph->host_only[i] = 1;
-            ph->host_result[i] = empty_score(j.problem, j.n1, j.n2, j.flags);
+            ph->host_result[i] = wf_empty_score(j.problem, j.n1, j.n2, j.flags);
             p.ngroups = 0;
             continue;
         }
@@ -453,23 +476,9 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         const wf_job &j = jobs[i];
         WfPairDev &p = ph->pairs[i];
         int32_t mode = 0;
-        const int st = wf_encode_pair(j.problem, j.a, j.n1, j.b, j.n2, ph->codes.data() + p.a_off,
-                                      ph->codes.data() + p.b_off, &mode);
+        const int st = wf_encode_job(j, tile ? tile->flags : 0, perm4, ph->codes.data() + p.a_off,
+                                     ph->codes.data() + p.b_off, &mode);
         if (st != WF_OK) return st;
-        if (j.problem == WF_ED && (j.flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
-        if (tile && j.problem == WF_ED && (tile->flags & WF_TILE_ED_TEXTBOOK)) mode |= WFK_MODE_TEXTBOOK;
-        // SW with at most 4 symbols on a cell engine with two rows per lane:
-        // byte row tables and plain codes (WFK_MODE_PERM4).
-        if (perm4 && j.problem == WF_SW && !(mode & WFK_MODE_CMP)) {
-            bool small = true;
-            for (int64_t k = 0; small && k < j.n1; ++k) small = ph->codes[p.a_off + k] <= 12;
-            for (int64_t k = 0; small && k < j.n2; ++k) small = ph->codes[p.b_off + k] <= 12;
-            if (small) {
-                for (int64_t k = 0; k < j.n1; ++k) ph->codes[p.a_off + k] >>= 2;
-                for (int64_t k = 0; k < j.n2; ++k) ph->codes[p.b_off + k] >>= 2;
-                mode = (mode & WFK_MODE_TEXTBOOK) | WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD;
-            }
-        }
         p.mode = mode;
         // The bit-parallel kernel indexes a 32-row table with the codes.
         if (bitpar && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;

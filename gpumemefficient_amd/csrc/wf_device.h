@@ -41,6 +41,11 @@ struct WfPairDev {
     // entry, ED: Pv, Mv; LCS: V, 0), at entry (strip * waves + wave) * 64 +
     // lane; -1: not stored.
     int64_t col_off = -1;
+    // Tag of this pair's strip-boundary words; 0: the launch epoch.  The
+    // resident server, whose one launch outlives many pairs in recycled
+    // memory, gives every request a tag of its own.
+    uint32_t tag = 0;
+    int32_t pad = 0;
 };
 
 // A pair computed as two column halves (bit-parallel engine, LCS and ED):
@@ -90,6 +95,71 @@ struct WfKernelArgs {
                              // every launch, so graph replays get fresh tags)
     int32_t *table;          // table-dump kernel only: H(y+1, x+1) of pair 0 at [y * n1 + x]
     uint32_t *colw;          // bit-parallel engine: last-column entries, two words each (WfPairDev::col_off)
+};
+
+// ----------------------------------------------------------------------------
+// Resident server: one persistent launch of the lock-step engine serves every
+// request for the server's lifetime (wf_serve_engine in wf_kernels.hip).
+// Requests enter through a ring in pinned host memory; the kernel's waiting
+// workgroups take them in (copy the codes to HBM, write the pair
+// descriptor, append the pair's strips to a task ring in HBM), run strips in
+// task order, and the last strip of a pair writes a completion record to a
+// second pinned ring.
+// ----------------------------------------------------------------------------
+
+// Submission ring entry (pinned host memory, written by the host).
+struct WfSubmitDev {
+    WfPairDev pd;        // a_off/b_off/bnd_off in the server's device arena, tag, ngroups
+    int64_t task_base;   // index of the pair's first strip in the task ring
+    int64_t stage_off;   // byte offset of the pair's codes in the staging buffer (same layout as from a_off)
+    int64_t code_bytes;  // bytes from a_off to the end of b's codes, a multiple of 16
+    int32_t slot;        // pair slot: descriptor, result and done counter
+    int32_t pad;
+    uint64_t seq;        // submission index + 1 once the entry is complete (written last)
+};
+
+// Task ring entry (HBM): strip `group` of the pair in `slot`; seq = task
+// index + 1 once valid.
+struct WfTaskQDev {
+    int32_t slot;
+    int32_t group;
+    uint64_t seq;
+};
+
+// Completion ring entry (pinned host memory, written by the device).
+struct WfCompDev {
+    int32_t last, maxs, status, slot;
+    uint64_t seq;        // completion index + 1 once the entry is complete (written last)
+};
+
+// Device-side control block (HBM).
+struct WfServeCtl {
+    uint64_t head;       // next task index to claim
+    uint64_t sub_head;   // next submission to take in
+    uint64_t comp_tail;  // next completion record
+    int32_t poller;      // 1 while a workgroup holds the lease to poll the host ring
+    int32_t stop;        // copy of the host's stop flag
+};
+
+// Host-side control block (pinned host memory).
+struct WfHostCtl {
+    uint64_t sub_tail;   // submissions published
+    int32_t stop;        // set by the host to end the launch once idle
+    int32_t pad;
+};
+
+struct WfServeArgs {
+    WfKernelArgs ka;            // pairs, codes, bnd, res: the server's tables; timeout
+    WfTaskQDev *taskq;
+    int64_t taskq_cap;
+    const WfSubmitDev *subq;    // pinned host
+    int64_t subq_cap;
+    const uint8_t *staging;     // pinned host
+    WfCompDev *compq;           // pinned host
+    int64_t compq_cap;
+    WfServeCtl *ctl;
+    const WfHostCtl *hctl;      // pinned host
+    int32_t *pair_done;         // strips finished, per slot
 };
 
 // In-place table stencils swept with the same strip schedule (wf_stencil.hip):

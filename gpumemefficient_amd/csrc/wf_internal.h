@@ -96,6 +96,14 @@ struct WfPlanHost {
 bool wf_bitpar_eligible(const wf_job *jobs, int64_t njobs);
 int wf_encode_pair(int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,
                    uint8_t *out_a, uint8_t *out_b, int32_t *mode);
+// Codes and WFK_MODE_* bits of one job for a cell or bit-parallel engine:
+// symbol encoding, the ED row-0 flag (job or tile flags) and, with `perm4`,
+// SW's byte row tables for pairs of at most 4 symbols.
+int wf_encode_job(const wf_job &j, int32_t tile_flags, bool perm4, uint8_t *a_out, uint8_t *b_out, int32_t *mode);
+// Score of a pair with an empty sequence (the reference boundaries).
+int32_t wf_empty_score(int problem, int64_t n1, int64_t n2, int flags);
+// Byte row tables for SW on cell engines with two rows per lane (WFDP_PERM=0: off).
+bool wf_perm4_enabled();
 const WfVariant *wf_pick_variant(struct wf_ctx *ctx, const wf_tile *tile, const wf_job *jobs, int64_t njobs,
                                  int64_t max_rows, int64_t total_cells);
 void wf_tune_key(const wf_job *jobs, int64_t njobs, int *problem, int *batch, int *bucket);

@@ -365,6 +365,210 @@ __global__ __launch_bounds__(256) void wf_engine_table(WfKernelArgs ka)
     engine_body<2, 4, 32, true>(ka);
 }
 
+// ----------------------------------------------------------------------------
+// Resident server engine: one persistent launch of the lock-step engine
+// serves every request of a server (structures in wf_device.h).
+//
+// Thread 0 of a free workgroup claims the next task index and waits for it
+// to become valid.  While it waits it may take the poll lease (one
+// workgroup at a time reads the host's submission ring, so the waiting grid
+// does not flood the host link); a new submission is then taken in by that
+// whole workgroup: the pair's codes are copied from pinned host memory to
+// HBM, its descriptor, result and done counter are set up, and its strips
+// are appended to the task ring in order, each made valid after a release
+// fence.  Strips are claimed in task order, so a strip only waits on strips
+// already claimed by running workgroups, as in a plan launch.  A strip runs
+// the ordinary lock-step sweep with the pair's own boundary tag; the last
+// strip of a pair to finish writes the completion record to pinned host
+// memory.  The launch ends when the host sets its stop flag (only when no
+// request is in flight).
+// ----------------------------------------------------------------------------
+enum { kServeRun = 0, kServeIntake = 1, kServeExit = 2 };
+
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_system(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Take submission `s` in (whole workgroup).
+__device__ void serve_intake(const WfServeArgs &sa, int64_t s)
+{
+    // (Raw storage: the descriptor type has default member initializers,
+    // which __shared__ variables cannot take.)
+    __shared__ __attribute__((aligned(16))) unsigned char se_raw[sizeof(WfSubmitDev)];
+    WfSubmitDev &se = *reinterpret_cast<WfSubmitDev *>(se_raw);
+    const WfSubmitDev *e = sa.subq + (s % sa.subq_cap);
+    if (threadIdx.x == 0) {
+        // The host writes the entry before it publishes its tail; the check
+        // of the entry's own sequence number is bounded all the same.
+        const int64_t t0 = now_ticks();
+        while (ld_system(&e->seq) != (uint64_t)s + 1 && now_ticks() - t0 < sa.ka.timeout_ticks)
+            __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        se = *e;
+    }
+    __syncthreads();
+    const int slot = se.slot;
+    // Codes: one contiguous, 16-byte aligned run from a_off.
+    const uint4 *src = reinterpret_cast<const uint4 *>(sa.staging + se.stage_off);
+    uint4 *dst = reinterpret_cast<uint4 *>(const_cast<uint8_t *>(sa.ka.codes) + se.pd.a_off);
+    const int64_t n16 = se.code_bytes / 16;
+    for (int64_t i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+    if (threadIdx.x == 0) {
+        const_cast<WfPairDev *>(sa.ka.pairs)[slot] = se.pd;
+        sa.ka.res[slot] = WfResultDev{0, 0, 0, 0};
+        sa.pair_done[slot] = 0;
+    }
+    for (int g = threadIdx.x; g < se.pd.ngroups; g += blockDim.x) {
+        WfTaskQDev *t = sa.taskq + ((se.task_base + g) % sa.taskq_cap);
+        t->slot = slot;
+        t->group = g;
+    }
+    // One release for the whole workgroup's stores (the barrier orders them
+    // before it), then the strips become valid.
+    __syncthreads();
+    if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    for (int g = threadIdx.x; g < se.pd.ngroups; g += blockDim.x) {
+        const int64_t idx = se.task_base + g;
+        __hip_atomic_store(&sa.taskq[idx % sa.taskq_cap].seq, (uint64_t)idx + 1, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+}
+
+// After the last strip of the pair in `slot` (thread 0).
+__device__ void serve_complete(const WfServeArgs &sa, int slot, int ngroups)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const int done = atomicAdd(sa.pair_done + slot, 1);
+    if (done != ngroups - 1) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const WfResultDev r = sa.ka.res[slot];
+    const uint64_t k = atomicAdd(reinterpret_cast<unsigned long long *>(&sa.ctl->comp_tail), 1ULL);
+    WfCompDev *c = sa.compq + (k % sa.compq_cap);
+    __hip_atomic_store(&c->last, r.last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&c->maxs, r.maxs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&c->status, r.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&c->slot, slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __hip_atomic_store(&c->seq, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int R, int W, int K>
+__global__ __launch_bounds__(64 * W, (lockstep_min_blocks<R, W>())) void wf_serve_engine(WfServeArgs sa)
+{
+    constexpr int AW = EngineCfg<R, W, K>::AW;
+    __shared__ WgShared<W, K, AW> sh;
+    __shared__ int64_t act[2];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int64_t idx = -1;     // thread 0: claimed task index not yet run
+    bool lease = false;   // thread 0: holds the poll lease
+    for (;;) {
+        if (threadIdx.x == 0) {
+            if (idx < 0) idx = (int64_t)atomicAdd(reinterpret_cast<unsigned long long *>(&sa.ctl->head), 1ULL);
+            // Waiting workgroups back off exponentially (up to about 3.4 us
+            // between polls) and look at the stop flag and the lease only
+            // every 8th poll: the grid's idle polling must not load the
+            // memory system the running strips use.
+            int backoff = 1;
+            for (int it = 0;; ++it) {
+                const WfTaskQDev *t = sa.taskq + (idx % sa.taskq_cap);
+                if (ld_agent(&t->seq) == (uint64_t)idx + 1) {
+                    act[0] = kServeRun;
+                    act[1] = idx;
+                    idx = -1;
+                    break;
+                }
+                const bool rare = (it & 7) == 0;
+                if (rare && __hip_atomic_load(&sa.ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    act[0] = kServeExit;
+                    break;
+                }
+                if (!lease && rare &&
+                    __hip_atomic_load(&sa.ctl->poller, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    lease = atomicCAS(&sa.ctl->poller, 0, 1) == 0;
+                if (lease) {
+                    backoff = 1;  // the lease holder polls the host at full rate
+                    if (__hip_atomic_load(&sa.hctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+                        __hip_atomic_store(&sa.ctl->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        continue;
+                    }
+                    const uint64_t tail = ld_system(&sa.hctl->sub_tail);
+                    const uint64_t sub = ld_agent(&sa.ctl->sub_head);
+                    if (sub < tail) {
+                        // Only the lease holder advances sub_head.
+                        __hip_atomic_store(&sa.ctl->sub_head, sub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&sa.ctl->poller, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        lease = false;
+                        act[0] = kServeIntake;
+                        act[1] = (int64_t)sub;
+                        break;
+                    }
+                }
+                // s_sleep takes an immediate: the backoff picks one.
+                switch (backoff) {
+                case 1: __builtin_amdgcn_s_sleep(1); break;
+                case 2: __builtin_amdgcn_s_sleep(2); break;
+                case 4: __builtin_amdgcn_s_sleep(4); break;
+                case 8: __builtin_amdgcn_s_sleep(8); break;
+                case 16: __builtin_amdgcn_s_sleep(16); break;
+                case 32: __builtin_amdgcn_s_sleep(32); break;
+                case 64: __builtin_amdgcn_s_sleep(64); break;
+                default: __builtin_amdgcn_s_sleep(127); break;
+                }
+                if (backoff < 128) backoff *= 2;
+            }
+            if (lease && act[0] != kServeIntake) {
+                __hip_atomic_store(&sa.ctl->poller, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                lease = false;
+            }
+            // Everything the strip reads was written before its task became
+            // valid (intake's release fence).
+            if (act[0] == kServeRun) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+        const int a = __builtin_amdgcn_readfirstlane((int)act[0]);
+        const int64_t arg = act[1];
+        __syncthreads();
+        if (a == kServeExit) break;
+        if (a == kServeIntake) {
+            serve_intake(sa, arg);
+            continue;
+        }
+        const WfTaskQDev td = sa.taskq[arg % sa.taskq_cap];
+        const WfPairDev pd = sa.ka.pairs[td.slot];
+        if (threadIdx.x == 0) sh.abort_at = 0x7fffffff;
+        __syncthreads();
+        set_wave_priority(pd.prio);
+        const bool cmp = (pd.mode & WFK_MODE_CMP) != 0;
+        const uint32_t tag = pd.tag;
+        switch (pd.prob * 2 + (cmp ? 1 : 0)) {
+        case WFK_LCS * 2 + 0: run_group<WFK_LCS, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
+        case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
+        case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
+        case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
+        case WFK_SW * 2 + 0:
+            if constexpr (R == 2) {
+                if (pd.mode & WFK_MODE_PERM4) {
+                    run_group<WFK_SWP, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
+                    break;
+                }
+            }
+            run_group<WFK_SW, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
+            break;
+        default: run_group<WFK_SW, true, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) serve_complete(sa, td.slot, pd.ngroups);
+    }
+}
+
 // Zero progress counters, results and claim counters before a launch (a
 // kernel rather than hipMemsetAsync so the sequence can be graph-captured).
 // Also advances the launch epoch that tags strip-boundary values.
@@ -414,6 +618,21 @@ extern "C" hipError_t wf_launch_table(int grid, const WfKernelArgs *ka, hipStrea
     void *args[] = {const_cast<WfKernelArgs *>(ka)};
     return hipLaunchKernel(reinterpret_cast<const void *>(&wfk::wf_engine_table), dim3(grid), dim3(256), args, 0,
                            stream);
+}
+
+// The resident server engine: lock-step <2, 4, 64> (four-wave workgroups of
+// 512-row strips, byte row tables for SW of at most 4 symbols).
+extern "C" const void *wf_serve_kernel(int *block)
+{
+    *block = 64 * 4;
+    return reinterpret_cast<const void *>(&wfk::wf_serve_engine<2, 4, 64>);
+}
+
+extern "C" hipError_t wf_launch_serve(int grid, const WfServeArgs *sa, hipStream_t stream)
+{
+    void *args[] = {const_cast<WfServeArgs *>(sa)};
+    return hipLaunchKernel(reinterpret_cast<const void *>(&wfk::wf_serve_engine<2, 4, 64>), dim3(grid), dim3(64 * 4),
+                           args, 0, stream);
 }
 
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,

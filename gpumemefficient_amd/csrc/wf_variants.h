@@ -40,6 +40,10 @@ extern "C" hipError_t wf_launch_bitpar_combine(const WfSplitDev *splits, int nsp
                                                WfResultDev *res, hipStream_t stream);
 extern "C" hipError_t wf_launch_engine(const WfVariant *v, int grid, const WfKernelArgs *ka,
                                        hipStream_t stream);
+// Resident server engine (lock-step <2, 4, 64>): the kernel (for occupancy
+// queries, with its block size) and its launch.
+extern "C" const void *wf_serve_kernel(int *block);
+extern "C" hipError_t wf_launch_serve(int grid, const WfServeArgs *sa, hipStream_t stream);
 
 // Stencil sweep (wf_stencil.hip): op WFS_SAT / WFS_SOR, waves 1, 2 or 4 per
 // workgroup; grid <= 0 picks one workgroup per strip up to the resident limit.

@@ -36,10 +36,15 @@ def _comp(c: wf_completion) -> Completion:
 
 class Server:
     def __init__(self, device: int = 0, streams: int = 4, max_batch: int = 512, hbm_budget: int = 0,
-                 queue_capacity: int = 0, per_request: bool = False):
+                 queue_capacity: int = 0, per_request: bool = False, resident: bool = False):
+        """per_request: one launch per request (baseline); resident: one
+        persistent launch for the server's lifetime that takes requests in
+        as they arrive (device-resident task queue); else launches of the
+        admitted requests on `streams` slots."""
         self._lib = _lib.load()
         self._h = C.c_void_p()
-        opts = wf_server_opts(streams, max_batch, hbm_budget, queue_capacity, 1 if per_request else 0)
+        opts = wf_server_opts(streams, max_batch, hbm_budget, queue_capacity, 1 if per_request else 0,
+                              1 if resident else 0)
         check(self._lib.wf_server_create(device, C.byref(opts), C.byref(self._h)), "wf_server_create")
         self._keep = []
 

@@ -270,6 +270,8 @@ void wf_plan_destroy(wf_plan *plan);
  * device footprint fits the HBM budget and launches every admitted request
  * not yet on the device as one persistent launch on the next free stream,
  * so requests that arrive while others run overlap with them on the device.
+ * In resident mode one persistent launch lives as long as the server and
+ * takes admitted requests in as they come (device-resident task queue).
  * No counterpart in the reference.
  * ---------------------------------------------------------------------- */
 typedef struct wf_server wf_server;
@@ -280,6 +282,10 @@ typedef struct {
     int64_t hbm_budget;     /* device bytes the server may hold; 0 = half of free memory */
     int32_t queue_capacity; /* queued requests before submit returns WF_ERR_AGAIN; 0 = 2^20 */
     int32_t per_request;    /* 1: one launch per request on one stream (baseline) */
+    int32_t resident;       /* 1: one persistent launch for the server's lifetime: requests
+                               are handed to it through pinned host memory, their strips
+                               queued in HBM and run as workgroups free up (no launch per
+                               batch); `streams` and `max_batch` do not apply */
 } wf_server_opts;
 
 typedef struct {

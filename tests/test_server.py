@@ -48,11 +48,18 @@ def test_serving_trace_generator():
     assert all(np.array_equal(x.a, y.a) and x.arrival_s == y.arrival_s for x, y in zip(reqs, again))
 
 
+MODES = [pytest.param(False, id="slots"), pytest.param(True, id="resident")]
+
+
 @pytest.mark.gpu
-def test_server_results_and_alone_vs_mix():
+@pytest.mark.parametrize("resident", MODES)
+def test_server_results_and_alone_vs_mix(resident):
+    """Every result equals the oracle's, submitted alone (one at a time) and
+    in a mix of 120 overlapping requests; with the resident engine the mix
+    runs inside one persistent launch that took each request in as it came."""
     jobs = _jobs(31, 120, 50, 3000)
     want = [_want(j) for j in jobs]
-    with Server(0, streams=4) as srv:
+    with Server(0, streams=4, resident=resident) as srv:
         # Alone: one request at a time.
         alone = []
         for j in jobs[:20]:
@@ -76,13 +83,14 @@ def test_server_results_and_alone_vs_mix():
 
 
 @pytest.mark.gpu
-def test_server_admission_by_budget():
+@pytest.mark.parametrize("resident", MODES)
+def test_server_admission_by_budget(resident):
     """A tight HBM budget forces requests to wait for earlier ones to
     finish; all still complete correctly.  A request that can never fit
     fails with WF_ERR_NOMEM instead of blocking the queue."""
     jobs = _jobs(32, 30, 1000, 4000)
     want = [_want(j) for j in jobs]
-    with Server(0, streams=2, hbm_budget=600_000) as srv:
+    with Server(0, streams=2, hbm_budget=600_000, resident=resident) as srv:
         tickets = [srv.submit(j) for j in jobs]
         big = Job("SW", np.zeros(40000, np.int32), np.zeros(40000, np.int32))
         tb = srv.submit(big)
@@ -92,11 +100,14 @@ def test_server_admission_by_budget():
     for t, w in zip(tickets, want):
         assert got[t].status == 0 and (got[t].last, got[t].max_score) == w
     assert got[tb].status == _lib.WF_ERR_NOMEM
-    assert st["launches"] >= 3  # could not all go in one launch
+    if not resident:
+        assert st["launches"] >= 3  # could not all go in one launch
+    assert st["hbm_used"] == 0
 
 
 @pytest.mark.gpu
-def test_server_replay_poisson():
+@pytest.mark.parametrize("resident", MODES)
+def test_server_replay_poisson(resident):
     reqs = serving_requests(2000.0, 150, seed=7)
     rng = np.random.default_rng(1)
     jobs = []
@@ -104,7 +115,7 @@ def test_server_replay_poisson():
         a, b = r.a[:2500], r.b[:2500]
         jobs.append(Job(r.problem, a, b))
     arrivals = [r.arrival_s for r in reqs]
-    with Server(0) as srv:
+    with Server(0, resident=resident) as srv:
         comps = srv.replay(jobs, arrivals)
     for c, j, t in zip(comps, jobs, arrivals):
         assert c.status == 0
@@ -126,7 +137,8 @@ def test_server_per_request_baseline_matches():
 
 
 @pytest.mark.gpu
-def test_server_refuses_a_bad_request_and_serves_the_rest():
+@pytest.mark.parametrize("resident", MODES)
+def test_server_refuses_a_bad_request_and_serves_the_rest(resident):
     """A request of 300 distinct symbols (more than the one-byte codes hold)
     is refused at submission with WF_ERR_ALPHABET; the good requests around
     it complete exactly, and the server keeps serving afterwards."""
@@ -134,7 +146,7 @@ def test_server_refuses_a_bad_request_and_serves_the_rest():
     jobs = _jobs(34, 40, 50, 2000)
     want = [_want(j) for j in jobs]
     bad = Job("SW", np.arange(300, dtype=np.int32), np.arange(300, dtype=np.int32)[::-1].copy())
-    with Server(0, streams=2) as srv:
+    with Server(0, streams=2, resident=resident) as srv:
         tickets = [srv.submit(j) for j in jobs[:20]]
         with pytest.raises(WfError) as ei:
             srv.submit(bad)
@@ -148,3 +160,29 @@ def test_server_refuses_a_bad_request_and_serves_the_rest():
     for t, w in zip(tickets, want):
         assert got[t].status == 0 and (got[t].last, got[t].max_score) == w
     assert len(more) == 1 and more[0].ticket == again and (more[0].last, more[0].max_score) == want[0]
+
+
+@pytest.mark.gpu
+def test_resident_server_empty_and_textbook_and_large_alphabets():
+    """Requests with an empty side (answered from the boundary), ED with the
+    textbook row 0, and pairs of 40 and 256 distinct symbols (compared
+    codes) through the resident engine, against the oracle; then a second
+    server in the same process (a fresh launch) gives the same answers."""
+    rng = np.random.default_rng(35)
+    jobs = [Job("ED", np.arange(7, dtype=np.int32), np.zeros(0, np.int32)),
+            Job("ED", np.zeros(0, np.int32), np.arange(5, dtype=np.int32)),
+            Job("SW", np.zeros(0, np.int32), np.zeros(0, np.int32)),
+            Job("ED", rng.integers(0, 20, 900, dtype=np.int32), rng.integers(0, 20, 1100, dtype=np.int32),
+                ed_textbook=True),
+            Job("ED", rng.integers(0, 40, 700, dtype=np.int32), rng.integers(0, 40, 600, dtype=np.int32)),
+            Job("SW", rng.integers(0, 256, 800, dtype=np.int32), rng.integers(0, 256, 900, dtype=np.int32)),
+            Job("LCS", rng.integers(-5, 5, 1300, dtype=np.int32), rng.integers(-5, 5, 513, dtype=np.int32))]
+    want = [O.score(j.problem, j.a, j.b, textbook=j.ed_textbook) for j in jobs]
+    for _ in range(2):
+        with Server(0, resident=True) as srv:
+            tickets = [srv.submit(j) for j in jobs]
+            srv.wait_idle(60)
+            got = {c.ticket: c for c in srv.poll()}
+        for t, w, j in zip(tickets, want, jobs):
+            assert got[t].status == 0
+            assert (got[t].last, got[t].max_score) == (w[0], w[1] if j.problem == "SW" else 0), j.problem
