@@ -399,15 +399,20 @@ def main() -> int:
         extra["vs_naive_lcs_8192"] = naive
         p.close()
         # Served mixed workload (Poisson arrivals, 50% ED / 50% SW, mixed
-        # lengths), continuous batching against one launch per request.
+        # lengths): continuous batching on stream slots, the resident
+        # persistent launch, and one launch per request.
         from gpumemefficient_amd.serve_bench import run_rate
-        serve = {"requests_per_rate": args.serve_requests, "continuous": [], "per_request": []}
+        serve = {"requests_per_rate": args.serve_requests, "continuous": [], "resident": [],
+                 "per_request": []}
         for rate in (1000.0, 2000.0, 5000.0, 10000.0, 20000.0, 50000.0):
             serve["continuous"].append(run_rate(rate, args.serve_requests, False))
+            serve["resident"].append(run_rate(rate, args.serve_requests, False, resident=True))
         serve["per_request"].append(run_rate(50000.0, args.serve_requests, True))
         sat_c = serve["continuous"][-1]["gcups"]
+        sat_r = serve["resident"][-1]["gcups"]
         sat_p = serve["per_request"][-1]["gcups"]
         serve["saturated_speedup"] = round(sat_c / sat_p, 2) if sat_p else None
+        serve["saturated_speedup_resident"] = round(sat_r / sat_p, 2) if sat_p else None
         extra["serving"] = serve
         extra["stencil_32768"] = stencil_measure(ctx)
         extra["lcs_32768_reference_input_drop_in"] = lcs_published_measure()

@@ -37,7 +37,11 @@ namespace wfk {
 struct TraceRec {
     int64_t t_begin, t_start, t_end;   // s_memrealtime (100 MHz)
     int64_t wait_cyc, tot_cyc;         // s_memtime cycles
-    int32_t nwait, hwid, xcc, pad;
+    int32_t nwait, hwid, xcc, nwait_out;
+    int64_t t_ck[16];                  // when steps 16384 * (k + 1) were done
+    int64_t c_ck[16];                  // the same in s_memtime cycles
+    int64_t w_ck[16];                  // wait cycles so far
+    int32_t n_ck[16];                  // waits so far
 };
 constexpr int kTraceTasks = 8192, kTraceWaves = 9;
 __device__ TraceRec g_trace[kTraceTasks * kTraceWaves];
@@ -91,11 +95,6 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 // from moving the accesses across the counter (a signal fence).  Building
 // with -DWFK_LDS_HW_FENCE=1 uses s_waitcnt-based workgroup fences instead
 // (which drain the read pipeline at every group boundary).
-// Step of a group's last block after which the counters for the next check
-// are loaded.
-#ifndef WFK_CNT_AT
-#define WFK_CNT_AT (kB / 2 - 1)
-#endif
 // Sleep (s_sleep units of 64 clocks) of the output wave between polls of
 // the last compute wave's progress, and of the input wave while it has no
 // room to stage into.
@@ -104,6 +103,10 @@ __device__ __forceinline__ void cnt_store(int *p, int v)
 #endif
 #ifndef WFK_IN_SLEEP
 #define WFK_IN_SLEEP 16
+#endif
+// Polls of a waiting compute wave before it sleeps between polls.
+#ifndef WFK_SPIN_FREE
+#define WFK_SPIN_FREE 64
 #endif
 
 #ifndef WFK_LDS_HW_FENCE
@@ -161,9 +164,11 @@ __device__ __forceinline__ void load_block(Block &b, const int *ring_prev, const
 // operands of g's first block (LDS executes one wave's accesses in order, so
 // operands read after a counter that shows the producer's stores see those
 // stores), and checked when the group starts.  A short counter sends the
-// wave to `wait_next` (out of line), after which it re-reads the block's
-// operands.  A wave thus trails its producer by 64 + G steps plus the LDS
-// latency, instead of 64 + 2G with a check one group ahead.  The group's
+// wave to a poll loop that re-reads the counters and the block's operands
+// together (one LDS round trip per poll; the operands are valid once the
+// counters read with them suffice), with `wait_slow` between polls.  A wave
+// thus trails its producer by 64 + G steps plus the LDS latency, instead of
+// 64 + 2G with a check one group ahead.  The group's
 // progress is published after its last block (publishing after every block
 // measured slower: the step is bound by the wave's instruction issue).  There is one load site per
 // block on the fast path, so the loop-carried block stays in the same VGPRs.
@@ -174,7 +179,7 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
                                           const int *ring_prev,
                                           int *ring_cur, int *dummy, const uint32_t *aw, Block &blk,
                                           const int *cin, const int *cout, int &vin, int &vout,
-                                          NeedIn &&need_in, NeedOut &&need_out, WaitFn &&wait_next,
+                                          NeedIn &&need_in, NeedOut &&need_out, WaitFn &&wait_slow,
                                           PubFn &&publish)
 {
     static_assert(U % G == 0 && G % kB == 0, "bad unroll");
@@ -202,13 +207,26 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
             // Global group index (the steady loop starts at multiples of
             // kU / G groups, so there it is known at compile time).
             const bool out_group = MASKED ? (gi % kOutEvery == 0) : ((q / G) % kOutEvery == 0);
-            if (out_group) short_ = short_ || vout < need_out(gi + kOutEvery - 1);
+            // Output room is checked for every group up to the next checked
+            // one (the groups in between do not look at vout).
+            const int gout = out_group ? gi + kOutEvery - 1 : gi;
+            if (out_group) short_ = short_ || vout < need_out(gout);
             if (__builtin_expect(__builtin_amdgcn_ballot_w64(short_) != 0, 0)) {
-                ok = ok && wait_next(gi);
+                for (int spins = 0;; ++spins) {
+                    vin = cnt_load(cin);
+                    vout = cnt_load(cout);
+                    lds_acquire();
 #pragma unroll
-                for (int i = 0; i < kB / 4; ++i) blk.inj[i] = *reinterpret_cast<const int4 *>(ib + q + 4 * i);
+                    for (int i = 0; i < kB / 4; ++i)
+                        blk.inj[i] = *reinterpret_cast<const int4 *>(ib + q + 4 * i);
 #pragma unroll
-                for (int u = 0; u < kB; ++u) blk.ca[u] = cp[q + u];
+                    for (int u = 0; u < kB; ++u) blk.ca[u] = cp[q + u];
+                    if (__builtin_amdgcn_ballot_w64(vin < need_in(gi) || vout < need_out(gout)) == 0) break;
+                    if (!wait_slow(spins)) {
+                        ok = false;
+                        break;
+                    }
+                }
             }
             lds_acquire();
         }
@@ -265,7 +283,8 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
     const int64_t tr_begin = __builtin_amdgcn_s_memrealtime();
     const int64_t tr_c0 = clock64();
     int64_t tr_start = 0, tr_wait = 0;
-    int tr_nwait = 0;
+    int tr_nwait = 0, tr_nwait_out = 0;
+    bool tr_out_short = false;
     auto tr_store = [&]() {
         if (lane == 0 && task < kTraceTasks) {
             TraceRec &r = g_trace[task * kTraceWaves + wave];
@@ -275,6 +294,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
             r.wait_cyc = tr_wait;
             r.tot_cyc = clock64() - tr_c0;
             r.nwait = tr_nwait;
+            r.nwait_out = tr_nwait_out;
             r.hwid = __builtin_amdgcn_s_getreg((31 << 11) | 4);
             r.xcc = __builtin_amdgcn_s_getreg((31 << 11) | 20);
         }
@@ -349,6 +369,9 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         auto ready = [&](int q) __attribute__((always_inline)) -> bool {
             const int vi = __builtin_amdgcn_readfirstlane(cnt_load(cin));
             const int vo = __builtin_amdgcn_readfirstlane(cnt_load(cout));
+#ifdef WFK_TRACE
+            tr_out_short = vi >= need_in(q) && vo < need_out(q);
+#endif
             return vi >= need_in(q) && vo >= need_out(q);
         };
         auto wait_ready = [&](int q) __attribute__((always_inline)) -> bool {
@@ -356,6 +379,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
 #ifdef WFK_TRACE
                 const int64_t c0 = clock64();
                 ++tr_nwait;
+                tr_nwait_out += tr_out_short ? 1 : 0;
                 struct Acc {
                     int64_t c0, *acc;
                     __device__ ~Acc() { *acc += clock64() - c0; }
@@ -365,7 +389,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
                 // LDS round trip is the natural poll period), then sleep.
                 const int64_t t0 = now_ticks();
                 for (int spins = 0;; ++spins) {
-                    if (spins >= 64) __builtin_amdgcn_s_sleep(1);
+                    if (spins >= WFK_SPIN_FREE) __builtin_amdgcn_s_sleep(1);
                     if (ready(q)) break;
                     if (__builtin_amdgcn_readfirstlane(
                             __hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ||
@@ -381,6 +405,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
 #ifdef WFK_TRACE
         tr_start = __builtin_amdgcn_s_memrealtime();
         tr_nwait = 0;
+        tr_nwait_out = 0;
         tr_wait = 0;
 #endif
         // Counters first, then the first block's operands (check on use).
@@ -394,11 +419,40 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         int prog = cnt_load(cmine);  // 0: counters are cleared before each strip
         auto publish = [&](int gi) __attribute__((always_inline)) {
             (void)gi;
+#ifdef WFK_TRACE
+            if ((((gi + 1) * G) & 16383) == 0 && (gi + 1) * G <= 16 * 16384 && lane == 0 && task < kTraceTasks)
+            {
+                g_trace[task * kTraceWaves + wave].t_ck[(gi + 1) * G / 16384 - 1] = __builtin_amdgcn_s_memrealtime();
+                g_trace[task * kTraceWaves + wave].c_ck[(gi + 1) * G / 16384 - 1] = clock64();
+                g_trace[task * kTraceWaves + wave].w_ck[(gi + 1) * G / 16384 - 1] = tr_wait;
+                g_trace[task * kTraceWaves + wave].n_ck[(gi + 1) * G / 16384 - 1] = tr_nwait;
+            }
+#endif
             prog = min(nsteps, prog + G);
             lds_release();
             cnt_store(cmine, prog);
         };
-        auto wait_next = [&](int gi) __attribute__((always_inline)) { return wait_ready(gi); };
+        // Between two polls of a short group: sleep once the wave has
+        // polled WFK_SPIN_FREE times, and look at the abort flag and the
+        // clock only every 16th poll (both cost a round trip of their own).
+        int64_t t_wait0 = 0;
+        auto wait_slow = [&](int spins) __attribute__((always_inline)) -> bool {
+#ifdef WFK_TRACE
+            if (spins == 0) ++tr_nwait;
+#endif
+            if (spins >= WFK_SPIN_FREE) __builtin_amdgcn_s_sleep(1);
+            if ((spins & 15) == 15) {
+                if (__builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(&sh.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+                    return false;
+                const int64_t t = now_ticks();
+                if (spins == 15)
+                    t_wait0 = t;
+                else if (t - t_wait0 > ka.timeout_ticks)
+                    return false;
+            }
+            return true;
+        };
         // Steps whose columns are valid for every lane run the unmasked body,
         // kU steps per loop iteration; the masked ones (lanes before column
         // 0 at the start, past column n1 - 1 at the end) run one group per
@@ -412,15 +466,15 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         for (; !aborted && q < ngr && !(steady_from(q) && (q * G) % kU == 0); ++q)
             aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
                 st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
-                need_in, need_out, wait_next, publish);
+                need_in, need_out, wait_slow, publish);
         for (; !aborted && steady_from(q); q += UG)
             aborted = !run_steps<P, CMP, R, G, kU, AW, RS, false, TRACK>(
                 st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
-                need_in, need_out, wait_next, publish);
+                need_in, need_out, wait_slow, publish);
         for (; !aborted && q < ngr; ++q)
             aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
                 st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
-                need_in, need_out, wait_next, publish);
+                need_in, need_out, wait_slow, publish);
         if (aborted) {
             // Mark the pair failed: in a pair's last strip no output wave
             // exists, and an input wave that has staged every column is gone,
@@ -540,9 +594,19 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         uint64_t t;
         uint32_t c;
         int base;
+#ifdef WFK_TRACE
+        int64_t t_issue;
+#endif
     };
+#ifdef WFK_TRACE
+    int64_t tr_lat[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tr_cols[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int tr_polls[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#endif
     auto issue = [&](Poll &p) __attribute__((always_inline)) {
         p.base = in_next;
+#ifdef WFK_TRACE
+        p.t_issue = __builtin_amdgcn_s_memrealtime();
+#endif
         const int x = in_next + lane;
         const bool in = x < n1;
         p.c = in ? (uint32_t)acodes[x] : 0u;
@@ -559,6 +623,14 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         const int ready = smask ? (int)__builtin_ctzll(smask) : 64;
         const int room = min(p0 + RS, pl - 63 - G + AW);
         const int limit = min(min(n1, p.base + ready), room);
+#ifdef WFK_TRACE
+        {
+            const int bk = min(7, p.base >> 14);
+            tr_lat[bk] += __builtin_amdgcn_s_memrealtime() - p.t_issue;
+            tr_polls[bk] += 1;
+            tr_cols[bk] += max(0, limit - in_next);
+        }
+#endif
         bool did = false;
         if (limit > in_next) {
             const int x = p.base + lane;
@@ -576,10 +648,13 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         if (in_next < n1) issue(p);
         return did;
     };
-    Poll pa, pb;
-    issue(pa);
-    if (has_prev) __builtin_amdgcn_s_sleep(16);  // about half a poll latency
-    issue(pb);
+    constexpr int NP = 2;  // four in flight measured slower (more, smaller bursts)
+    Poll pp[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        if (i > 0 && has_prev) __builtin_amdgcn_s_sleep(32 / NP);  // spread over a poll latency
+        issue(pp[i]);
+    }
     // Idle time only counts while the compute waves are not moving either,
     // i.e. while everything waits on the producing strip.
     int64_t t_idle = now_ticks();
@@ -587,9 +662,13 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
     while (in_next < n1) {
         int p0 = __builtin_amdgcn_readfirstlane(cnt_load(c0));
         int pl = __builtin_amdgcn_readfirstlane(cnt_load(clast));
-        bool did = examine(pa, p0, pl);
+        bool did = false;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            if (in_next >= n1) break;
+            did = examine(pp[i], p0, pl) || did;
+        }
         if (in_next >= n1) break;
-        did = examine(pb, p0, pl) || did;
         const bool moving = (p0 + pl != seen);
         seen = p0 + pl;
         if (did || moving) t_idle = now_ticks();
@@ -609,6 +688,12 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
     }
 #ifdef WFK_TRACE
     tr_store();
+    if (lane == 0 && task < kTraceTasks)
+        for (int k = 0; k < 8; ++k) {
+            g_trace[task * kTraceWaves + wave].w_ck[k] = tr_lat[k];
+            g_trace[task * kTraceWaves + wave].n_ck[k] = tr_polls[k];
+            g_trace[task * kTraceWaves + wave].c_ck[k] = tr_cols[k];
+        }
 #endif
 }
 
