@@ -20,7 +20,9 @@ One JSON line on stdout:
                (profiles/traffic.json), when measured on this kernel variant
   cpu_baseline the oracle (C restatement of SW.cpp) timed on this host on the
                same pair: one thread over the whole pair, plus the all-cores
-               anti-diagonal tile variant (thread count stated)
+               anti-diagonal tile variant (thread count stated), plus the
+               reference's CPU LCS program as it runs (the whole table) on
+               its own 2^15 input through oracle/bin/LCS --faithful
   extra        the other measurements: edit distance 65536^2 and LCS 8192^2
                single pairs; the 4096-pair SW batch (one persistent launch,
                eager and replayed from a hipGraph, tile from the shipped
@@ -246,7 +248,35 @@ def cpu_baseline(jobs, prob: str, budget_s: float = 20.0, min_s: float = 10.0):
             "all_cores": {"value": round(pcells / pdt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
                           "sample": (f"{prob} oracle over anti-diagonals of 256x256 tiles, {threads} threads, on the "
                                      f"whole {b.size}x{a.size} pair, {preps} run(s): {pcells:.3e} cells in "
-                                     f"{pdt:.1f} s")}}
+                                     f"{pdt:.1f} s")},
+            "faithful_lcs_32768": faithful_lcs_32768()}
+
+
+def faithful_lcs_32768() -> dict:
+    """The reference's CPU LCS program as it runs (the whole (n+1)^2 int32
+    table, filled in its loop order, timed around the DP call with
+    gettimeofday) on its own 2^15 input, through the drop-in CPU program
+    oracle/bin/LCS --faithful; its score must be the published 21469."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    exe = ROOT / "oracle" / "bin" / "LCS"
+    src = ROOT / "tests" / "golden" / "data" / "str1_2_15_str2_2_15.txt"
+    if not exe.exists() or not src.exists():
+        return {"skipped": "oracle/bin/LCS not built" if not exe.exists() else "input file missing"}
+    with tempfile.TemporaryDirectory() as td:
+        (Path(td) / "Data").mkdir()
+        shutil.copy(src, Path(td) / "Data" / src.name)
+        out = subprocess.run([str(exe), "15", "15", "--faithful"], cwd=td, capture_output=True, text=True,
+                             timeout=120).stdout
+    length = int(re.search(r"lcs length: (\d+)", out).group(1))
+    ms = float(re.search(r"execution time: ([0-9.e+-]+) ms", out).group(1))
+    assert length == 21469, length
+    return {"value": round(32768 * 32768 / ms / 1e6, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+            "ms": round(ms, 1), "lcs_length": length,
+            "sample": ("oracle/bin/LCS 15 15 --faithful on the reference's LCS/Data/str1_2_15_str2_2_15.txt "
+                       "(whole table, the reference program's memory behaviour), 1 run")}
 
 
 def single_pair_extra(ctx, name: str, peak: float, steps: int = 3) -> dict:
