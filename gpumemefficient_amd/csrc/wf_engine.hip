@@ -51,7 +51,7 @@ __device__ TraceRec g_trace[kTraceTasks * kTraceWaves];
 // granularity G): one loop back-edge per kU steps, the counter checks in
 // between are not-taken branches.  kB: steps per block of LDS operands.
 constexpr int kU = 32;
-constexpr int kB = 8;
+constexpr int kB = 8;  // a 4-step block (prefetch 4 steps ahead) measured slower: 7.76 against 7.71 ms
 
 template <int W, int G, int AW, int RS>
 struct AsyncShared {
