@@ -34,6 +34,7 @@ WF_TILE_NO_COPYBACK = 0x10
 WF_TILE_BITPAR = 0x20
 WF_TILE_BITPAR_STAGED = 0x40
 WF_TILE_HINT = 0x80
+WF_TILE_NO_MAX = 0x100
 ENGINES = ("lockstep", "async", "bitpar", "bitpar_staged")
 
 WF_TABLE_MAX_CELLS = 1 << 28

@@ -77,6 +77,7 @@ class Tile:
     cache: bool = False
     engine: str = ""      # "lockstep", "async", "bitpar", "bitpar_staged" or "" (default)
     hint: bool = False    # the fields are hints (WF_TILE_HINT): the planner picks the variant
+    no_max: bool = False  # SW: bottom-right cell only, no running maximum (WF_TILE_NO_MAX; max_score -1)
 
     def _c(self) -> _lib.wf_tile:
         flags = ((_lib.WF_TILE_ED_TEXTBOOK if self.ed_textbook else 0) | (_lib.WF_TILE_CACHE if self.cache else 0)
@@ -84,7 +85,8 @@ class Tile:
                  | (_lib.WF_TILE_ASYNC if self.engine == "async" else 0)
                  | (_lib.WF_TILE_BITPAR if self.engine == "bitpar" else 0)
                  | (_lib.WF_TILE_BITPAR_STAGED if self.engine == "bitpar_staged" else 0)
-                 | (_lib.WF_TILE_HINT if self.hint else 0))
+                 | (_lib.WF_TILE_HINT if self.hint else 0)
+                 | (_lib.WF_TILE_NO_MAX if self.no_max else 0))
         return _lib.wf_tile(self.tile_w, self.strip_rows, self.rows_per_lane, flags)
 
     @classmethod
@@ -93,7 +95,8 @@ class Tile:
                else "bitpar" if t.flags & _lib.WF_TILE_BITPAR
                else "bitpar_staged" if t.flags & _lib.WF_TILE_BITPAR_STAGED else "")
         return cls(t.tile_w, t.strip_rows, t.rows_per_lane, bool(t.flags & _lib.WF_TILE_ED_TEXTBOOK),
-                   bool(t.flags & _lib.WF_TILE_CACHE), eng)
+                   bool(t.flags & _lib.WF_TILE_CACHE), eng, bool(t.flags & _lib.WF_TILE_HINT),
+                   bool(t.flags & _lib.WF_TILE_NO_MAX))
 
 
 @dataclass
@@ -102,6 +105,7 @@ class Job:
     a: Sequence[int] | np.ndarray
     b: Sequence[int] | np.ndarray
     ed_textbook: bool = False
+    no_max: bool = False  # SW: bottom-right cell only (WF_TILE_NO_MAX as a job flag)
 
 
 class _JobArray:
@@ -114,8 +118,8 @@ class _JobArray:
         for i, j in enumerate(jl):
             a, b = _as_i32(j.a), _as_i32(j.b)
             self.arrays += [a, b]
-            self.c[i] = _lib.wf_job(_problem_id(j.problem), _lib.WF_TILE_ED_TEXTBOOK if j.ed_textbook else 0,
-                                    _ptr(a), a.size, _ptr(b), b.size)
+            flags = (_lib.WF_TILE_ED_TEXTBOOK if j.ed_textbook else 0) | (_lib.WF_TILE_NO_MAX if j.no_max else 0)
+            self.c[i] = _lib.wf_job(_problem_id(j.problem), flags, _ptr(a), a.size, _ptr(b), b.size)
         self.n = len(jl)
 
 
@@ -362,10 +366,13 @@ def SW(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,
        tX: int | None = None, tY: int | None = None, cache: bool = False) -> int:
     """Reference signature SW(n1, n2, arr1, arr2, paddX, paddY, table, tX, tY)
     (SmithWaterman/GPU_Hyperlane_Share.h:5).  Returns the bottom-right cell
-    ("last", SmithWaterman/SW.cpp:80)."""
+    ("last", SmithWaterman/SW.cpp:80); the running maximum, which the
+    reference does not compute, is skipped (no_max)."""
     del paddX, paddY, table
     a, b = _as_i32(arr1)[:n1], _as_i32(arr2)[:n2]
-    return default_context().align("SW", a, b, _tile_from_ref(tX, tY, cache)).last
+    t = _tile_from_ref(tX, tY, cache)
+    t.no_max = True
+    return default_context().align("SW", a, b, t).last
 
 
 def ED(n1: int, n2: int, arr1, arr2, paddX: int = 3, paddY: int = 1, table=None,

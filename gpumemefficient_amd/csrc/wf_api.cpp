@@ -216,7 +216,11 @@ const WfVariant *wf_pick_variant(wf_ctx *ctx, const wf_tile *tile, const wf_job 
         tile = &hinted;
     }
     const bool cache = tile && (tile->flags & WF_TILE_CACHE);
-    const bool explicit_tile = tile && (tile->tile_w > 0 || tile->strip_rows > 0 || tile->rows_per_lane > 0);
+    // The tuned table applies only when the caller chose neither a size nor
+    // an engine.
+    const int32_t engine_flags = WF_TILE_LOCKSTEP | WF_TILE_ASYNC | WF_TILE_BITPAR | WF_TILE_BITPAR_STAGED;
+    const bool explicit_tile = tile && (tile->tile_w > 0 || tile->strip_rows > 0 || tile->rows_per_lane > 0 ||
+                                        (tile->flags & engine_flags));
     if (!explicit_tile && !cache && ctx) {
         int p, b, bk, te, tr, tw, tk;
         wf_tune_key(jobs, njobs, &p, &b, &bk);
@@ -479,6 +483,10 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         const int st = wf_encode_job(j, tile ? tile->flags : 0, perm4, ph->codes.data() + p.a_off,
                                      ph->codes.data() + p.b_off, &mode);
         if (st != WF_OK) return st;
+        // SW pairs on the row tables may skip the running maximum (the cell
+        // engines have instances without it).
+        if (j.problem == WF_SW && !(mode & WFK_MODE_CMP) && ((j.flags | (tile ? tile->flags : 0)) & WF_TILE_NO_MAX))
+            mode |= WFK_MODE_NOMAX;
         p.mode = mode;
         // The bit-parallel kernel indexes a 32-row table with the codes.
         if (bitpar && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;
@@ -717,7 +725,7 @@ int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel
             continue;
         }
         o.last = r[i].last;
-        o.max_score = r[i].maxs;
+        o.max_score = (ph->pairs[i].mode & WFK_MODE_NOMAX) ? -1 : r[i].maxs;
         o.status = r[i].status ? WF_ERR_TIMEOUT : WF_OK;
         if (o.status) worst = o.status;
     }

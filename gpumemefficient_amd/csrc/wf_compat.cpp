@@ -144,7 +144,9 @@ static int run(int problem, int n1, int n2, const int *arr1, const int *arr2, in
                int paddY = 0, int *table = nullptr)
 {
     wf_ctx *c = ctx();
-    wf_tile t{tX, tY, 0, kPolicyFlags};
+    // The reference's SW() returns the bottom-right cell only, so the running
+    // maximum is not computed (WF_TILE_NO_MAX).
+    wf_tile t{tX, tY, 0, kPolicyFlags | (problem == WF_SW ? WF_TILE_NO_MAX : 0)};
     wf_result r{};
     const int32_t *a = reinterpret_cast<const int32_t *>(arr1), *b = reinterpret_cast<const int32_t *>(arr2);
     if (table && debug_table() && n1 == n2 && n1 > 0 && (int64_t)n1 * n2 <= WF_TABLE_MAX_CELLS) {

@@ -22,6 +22,8 @@ enum {
     // lock-step), two rows per lane: the codes are plain 0..3 and a lane's
     // row tables are bytes (see WFK_SWP in wf_cells.h).
     WFK_MODE_PERM4 = 16,
+    // SW without the running maximum (only the bottom-right cell).
+    WFK_MODE_NOMAX = 32,
 };
 enum { WFK_MODE_NEVER_SHIFT = 8 };
 

@@ -275,7 +275,7 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
 // Forced inline into the kernel: left as a call it takes the kernel
 // arguments by address and copies them (and the pair) to scratch memory,
 // which the steady loop then reloads.
-template <int P, bool CMP, int R, int W, int G, int AW, int RS>
+template <int P, bool CMP, int R, int W, int G, int AW, int RS, bool NOMAX = false>
 __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
                             AsyncShared<W, G, AW, RS> &sh, int lane, int wave, uint32_t epoch, int task)
 {
@@ -300,7 +300,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         }
     };
 #endif
-    constexpr bool TRACK = (P == WFK_SW || P == WFK_SWP);
+    constexpr bool TRACK = (P == WFK_SW || P == WFK_SWP) && !NOMAX;
     const int n1 = pd.n1, n2 = pd.n2;
     const int nsteps = n1 + 63;
     const bool has_prev = g > 0;
@@ -730,15 +730,25 @@ __global__ __launch_bounds__(64 * (W + 2)) void wf_engine_async(WfKernelArgs ka)
         case WFK_LCS * 2 + 1: async_strip<WFK_LCS, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         case WFK_ED * 2 + 0: async_strip<WFK_ED, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         case WFK_ED * 2 + 1: async_strip<WFK_ED, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
-        case WFK_SW * 2 + 0:
+        case WFK_SW * 2 + 0: {
+            // Instances without the running maximum for WFK_MODE_NOMAX.
+            const bool nomax = (pd.mode & WFK_MODE_NOMAX) != 0;
             if constexpr (R == 2) {
                 if (pd.mode & WFK_MODE_PERM4) {
-                    async_strip<WFK_SWP, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
+                    if (nomax)
+                        async_strip<WFK_SWP, false, R, W, G, AW, RS, true>(ka, pd, td.pair, td.group, sh, lane, wave,
+                                                                          epoch, t);
+                    else
+                        async_strip<WFK_SWP, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
                     break;
                 }
             }
-            async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
+            if (nomax)
+                async_strip<WFK_SW, false, R, W, G, AW, RS, true>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
+            else
+                async_strip<WFK_SW, false, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t);
             break;
+        }
         default: async_strip<WFK_SW, true, R, W, G, AW, RS>(ka, pd, td.pair, td.group, sh, lane, wave, epoch, t); break;
         }
         __syncthreads();

@@ -126,13 +126,13 @@ __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, in
 }
 
 // One strip (`g`) of one pair, swept by the whole workgroup.
-template <int P, bool CMP, int R, int W, int K, int AW, bool DUMP = false>
+template <int P, bool CMP, int R, int W, int K, int AW, bool DUMP = false, bool NOMAX = false>
 __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, int g,
                           WgShared<W, K, AW> &sh, int lane, int wave, uint32_t epoch)
 {
     constexpr int D = ChunkGeom<K>::D;
     constexpr int RS = ChunkGeom<K>::RS;
-    constexpr bool TRACK = (P == WFK_SW || P == WFK_SWP);
+    constexpr bool TRACK = (P == WFK_SW || P == WFK_SWP) && !NOMAX;
     const int n1 = pd.n1, n2 = pd.n2;
     const int64_t row0 = (int64_t)g * (W * 64 * R) + wave * (64 * R) + lane * R;
     const uint8_t *acodes = ka.codes + pd.a_off;
@@ -325,15 +325,32 @@ __device__ __forceinline__ void engine_body(const WfKernelArgs &ka)
         case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
-        case WFK_SW * 2 + 0:
+        case WFK_SW * 2 + 0: {
+            // Instances without the running maximum for WFK_MODE_NOMAX (the
+            // planner never sets it for the table dump).
+            const bool nomax = !DUMP && (pd.mode & WFK_MODE_NOMAX) != 0;
             if constexpr (R == 2) {
                 if (pd.mode & WFK_MODE_PERM4) {
+                    if constexpr (!DUMP) {
+                        if (nomax) {
+                            run_group<WFK_SWP, false, R, W, K, AW, false, true>(ka, pd, td.pair, td.group, sh, lane,
+                                                                               wave, epoch);
+                            break;
+                        }
+                    }
                     run_group<WFK_SWP, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
+                    break;
+                }
+            }
+            if constexpr (!DUMP) {
+                if (nomax) {
+                    run_group<WFK_SW, false, R, W, K, AW, false, true>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
                     break;
                 }
             }
             run_group<WFK_SW, false, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch);
             break;
+        }
         default: run_group<WFK_SW, true, R, W, K, AW, DUMP>(ka, pd, td.pair, td.group, sh, lane, wave, epoch); break;
         }
         __syncthreads();

@@ -220,7 +220,7 @@ static void finish_slot(wf_server *s, Slot &sl)
             c.last = sl.ph.host_result[i];
         } else {
             c.last = sl.pinned[i].last;
-            c.max_score = sl.pinned[i].maxs;
+            c.max_score = (sl.ph.pairs[i].mode & WFK_MODE_NOMAX) ? -1 : sl.pinned[i].maxs;
             c.status = sl.pinned[i].status ? WF_ERR_TIMEOUT : WF_OK;
         }
         c.t_submit = r.t_submit;

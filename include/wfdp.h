@@ -93,6 +93,14 @@ typedef struct {
                                    reference's tX, tY): the planner picks its own variant
                                    (autotune table, then defaults); every variant gives
                                    the same results */
+#define WF_TILE_NO_MAX 0x100    /* SW: compute only the bottom-right cell, which is all the
+                                   reference's SW() returns (SmithWaterman/SW.cpp:80,
+                                   GPU_Hyperlane_Share.cu:781), not the running maximum
+                                   (about 8% of the cell engines' instructions); such pairs
+                                   report max_score = -1.  Pairs with more symbols than the
+                                   kernels' row tables hold, and requests run by the
+                                   resident server's kernel, keep the maximum.  Also a job
+                                   flag (wf_job.flags). */
 
 typedef struct {
     int32_t last;      /* bottom-right cell: the score the reference prints */

@@ -186,3 +186,22 @@ def test_resident_server_empty_and_textbook_and_large_alphabets():
         for t, w, j in zip(tickets, want, jobs):
             assert got[t].status == 0
             assert (got[t].last, got[t].max_score) == (w[0], w[1] if j.problem == "SW" else 0), j.problem
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("resident", MODES)
+def test_server_no_max_requests(resident):
+    """SW requests with the no_max job flag: the bottom-right cell equals the
+    oracle's; launches on stream slots skip the maximum (max_score -1), the
+    resident kernel computes it anyway."""
+    rng = np.random.default_rng(34)
+    jobs = [Job("SW", rng.integers(0, 4, int(rng.integers(100, 2500)), dtype=np.int32),
+                rng.integers(0, 4, int(rng.integers(100, 2500)), dtype=np.int32), no_max=True) for _ in range(16)]
+    with Server(0, streams=2, resident=resident) as srv:
+        tickets = [srv.submit(j) for j in jobs]
+        srv.wait_idle(120)
+        got = {c.ticket: c for c in srv.poll()}
+    for t, j in zip(tickets, jobs):
+        last, mx = O.score("SW", j.a, j.b)
+        assert got[t].status == 0 and got[t].last == last
+        assert got[t].max_score == (mx if resident else -1)
