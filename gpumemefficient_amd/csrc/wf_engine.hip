@@ -51,24 +51,36 @@ __device__ TraceRec g_trace[kTraceTasks * kTraceWaves];
 // granularity G): one loop back-edge per kU steps, the counter checks in
 // between are not-taken branches.  kB: steps per block of LDS operands.
 constexpr int kU = 32;
+// The SW byte-table strips of the four-wave, G = 32 kernels (the single-pair
+// SW path) run 256 steps per iteration instead: one lap of the 256-slot
+// rings, so every ring address of an iteration is a constant offset and the
+// per-iteration scalar address arithmetic is amortised over 8 times the
+// steps (SW 131072^2 7.07 -> 6.75 ms).  Other strips keep kU, which keeps
+// the build time of the many variants in check.
+constexpr int kUMax = 256;
+template <int P, int W, int G>
+constexpr int steady_unroll()
+{
+    return (P == WFK_SWP && W == 4 && G == 32) ? kUMax : kU;
+}
 constexpr int kB = 8;  // a 4-step block (prefetch 4 steps ahead) measured slower: 7.76 against 7.71 ms
 
 template <int W, int G, int AW, int RS>
 struct AsyncShared {
     // Column codes, one per dword (a lane's codes of a block are then aligned
     // dword reads, which LDS serves faster than unaligned or byte reads):
-    // column x at slot (x + 64) mod AW, the first kU + kB slots mirrored at
-    // AW so that a lane's reads of one loop iteration never wrap.  First in
+    // column x at slot (x + 64) mod AW, the first kUMax + kB slots mirrored
+    // at AW so that a lane's reads of one loop iteration never wrap.  First in
     // the workgroup's LDS so that a lane's reads fit the 8-bit dword offsets
     // of ds_read2 without extra address arithmetic.
-    __attribute__((aligned(16))) uint32_t aw[AW + kU + kB];
+    __attribute__((aligned(16))) uint32_t aw[AW + kUMax + kB];
     // ring[0]: the strip's top input, staged by the input wave; ring[w+1]:
     // lane 63 of compute wave w, read by wave w+1 (w = W-1: by the output
     // wave).  Column x lives in slot (x + 64) mod RS.
     __attribute__((aligned(16))) int ring[W + 1][RS];
     // Lanes 0..62 store their bottom cell here so the capture is one
     // unmasked ds_write per step.
-    int dummy[W][64 + kU];
+    int dummy[W][64 + kUMax];
     // Progress counters, 64 copies each so a wave stores one value with an
     // unmasked ds_write:
     //   cnt[0]     columns staged in ring[0] and aw (input wave)
@@ -182,13 +194,20 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
                                           NeedIn &&need_in, NeedOut &&need_out, WaitFn &&wait_slow,
                                           PubFn &&publish)
 {
-    static_assert(U % G == 0 && G % kB == 0, "bad unroll");
+    static_assert(U % G == 0 && G % kB == 0 && U <= RS, "bad unroll");
     (void)ngr;
     const int s0 = q0 * G;
     // s0 is a multiple of U, so neither the ring nor the (mirrored) code
     // window wraps inside the iteration; only the next iteration's first
     // block is addressed afresh.
     const int *__restrict__ ib = ring_prev + ((s0 + 64) & (RS - 1));
+    // Lane 0's ring operands of the block at step s0 + qq (qq a multiple of
+    // kB): contiguous from ib unless the iteration is a whole lap of the
+    // ring (U == RS, s0 a multiple of RS), where each block sits at a
+    // constant offset.
+    auto ring_at = [&](int qq) __attribute__((always_inline)) -> const int * {
+        return (U == RS) ? ring_prev + ((64 + qq) & (RS - 1)) : ib + qq;
+    };
     const uint32_t *__restrict__ cp = aw + ((s0 - lane + 64) & (AW - 1));
     int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
     int *cap_last = (lane == 63) ? ring_cur + ((s0 + U) & (RS - 1)) : dummy + lane + (U - 1);
@@ -218,7 +237,7 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
                     lds_acquire();
 #pragma unroll
                     for (int i = 0; i < kB / 4; ++i)
-                        blk.inj[i] = *reinterpret_cast<const int4 *>(ib + q + 4 * i);
+                        blk.inj[i] = *reinterpret_cast<const int4 *>(ring_at(q) + 4 * i);
 #pragma unroll
                     for (int u = 0; u < kB; ++u) blk.ca[u] = cp[q + u];
                     if (__builtin_amdgcn_ballot_w64(vin < need_in(gi) || vout < need_out(gout)) == 0) break;
@@ -250,7 +269,7 @@ __device__ __forceinline__ bool run_steps(LaneState<R> &st, int q0, int ngr, int
         }
         // Past the last group the next block reads stale slots (in bounds)
         // that are never used.
-        const int *ip = (q + kB < U) ? ib + q + kB : ring_prev + ((s0 + U + 64) & (RS - 1));
+        const int *ip = (q + kB < U) ? ring_at(q + kB) : ring_prev + ((s0 + U + 64) & (RS - 1));
 #pragma unroll
         for (int i = 0; i < kB / 4; ++i) blk.inj[i] = *reinterpret_cast<const int4 *>(ip + 4 * i);
 #pragma unroll
@@ -457,18 +476,19 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         // kU steps per loop iteration; the masked ones (lanes before column
         // 0 at the start, past column n1 - 1 at the end) run one group per
         // iteration.  One body per loop keeps `blk` in fixed registers.
-        constexpr int UG = kU / G;
+        constexpr int KU = steady_unroll<P, W, G>();
+        constexpr int UG = KU / G;
         const int xlo = free_head ? -64 : 0;
         auto steady_from = [&](int q) __attribute__((always_inline)) {
-            return (free_head || q * G >= 63) && q * G + kU - 1 <= n1 - 1 && q + UG <= ngr;
+            return (free_head || q * G >= 63) && q * G + KU - 1 <= n1 - 1 && q + UG <= ngr;
         };
         int q = 0;
-        for (; !aborted && q < ngr && !(steady_from(q) && (q * G) % kU == 0); ++q)
+        for (; !aborted && q < ngr && !(steady_from(q) && (q * G) % KU == 0); ++q)
             aborted = !run_steps<P, CMP, R, G, G, AW, RS, true, TRACK>(
                 st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
                 need_in, need_out, wait_slow, publish);
         for (; !aborted && steady_from(q); q += UG)
-            aborted = !run_steps<P, CMP, R, G, kU, AW, RS, false, TRACK>(
+            aborted = !run_steps<P, CMP, R, G, KU, AW, RS, false, TRACK>(
                 st, q, ngr, lane, n1, xlo, sh.ring[w], sh.ring[w + 1], sh.dummy[w], sh.aw, blk, cin, cout, vin, vout,
                 need_in, need_out, wait_slow, publish);
         for (; !aborted && q < ngr; ++q)
@@ -587,7 +607,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         const uint32_t never =
             (P == WFK_SWP) ? Cell<WFK_SWP, false>::kNever : ((uint32_t)pd.mode >> WFK_MODE_NEVER_SHIFT) & 0x1FFu;
         sh.aw[lane] = never;
-        if (lane < kU + kB) sh.aw[AW + lane] = never;
+        if (lane < kUMax + kB) sh.aw[AW + lane] = never;
     }
     int in_next = 0;  // columns staged and published in cnt[0]
     struct Poll {
@@ -638,7 +658,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
                 sh.ring[0][(x + 64) & (RS - 1)] = has_prev ? (int)(uint32_t)p.t : topv;
                 const int slot = (x + 64) & (AW - 1);
                 sh.aw[slot] = p.c;
-                if (slot < kU + kB) sh.aw[AW + slot] = p.c;
+                if (slot < kUMax + kB) sh.aw[AW + slot] = p.c;
             }
             in_next = limit;
             lds_release();
