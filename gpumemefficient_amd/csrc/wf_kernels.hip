@@ -570,15 +570,24 @@ __global__ __launch_bounds__(64 * W, (lockstep_min_blocks<R, W>())) void wf_serv
         case WFK_LCS * 2 + 1: run_group<WFK_LCS, true, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
         case WFK_ED * 2 + 0: run_group<WFK_ED, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
         case WFK_ED * 2 + 1: run_group<WFK_ED, true, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
-        case WFK_SW * 2 + 0:
+        case WFK_SW * 2 + 0: {
+            const bool nomax = (pd.mode & WFK_MODE_NOMAX) != 0;
             if constexpr (R == 2) {
                 if (pd.mode & WFK_MODE_PERM4) {
-                    run_group<WFK_SWP, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
+                    if (nomax)
+                        run_group<WFK_SWP, false, R, W, K, AW, false, true>(sa.ka, pd, td.slot, td.group, sh, lane, wave,
+                                                                           tag);
+                    else
+                        run_group<WFK_SWP, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
                     break;
                 }
             }
-            run_group<WFK_SW, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
+            if (nomax)
+                run_group<WFK_SW, false, R, W, K, AW, false, true>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
+            else
+                run_group<WFK_SW, false, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag);
             break;
+        }
         default: run_group<WFK_SW, true, R, W, K, AW>(sa.ka, pd, td.slot, td.group, sh, lane, wave, tag); break;
         }
         __syncthreads();

@@ -119,6 +119,7 @@ struct ResidentReq {
     int64_t bnd_off = -1, bnd_bytes = 0;      // strip boundaries in the device boundary arena
     int64_t stage_off = -1, stage_bytes = 0;  // codes in pinned staging memory
     int32_t ngroups = 0;
+    bool nomax = false;  // SW without the running maximum: report max_score -1
 };
 
 // Resident mode: the persistent launch and everything it reads and writes.
@@ -543,7 +544,8 @@ static void resident_scheduler(wf_server *s)
             {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->used -= q.req.bytes;
-                push_done(s, q.req, c->last, c->maxs, st, s->now());
+                const bool nomax = q.req.problem == WF_SW && (q.nomax);
+                push_done(s, q.req, c->last, nomax ? -1 : c->maxs, st, s->now());
                 s->cv.notify_all();
             }
             q.req = Request();
@@ -605,6 +607,10 @@ static void resident_scheduler(wf_server *s)
             uint8_t *sa = r->staging + q.stage_off;
             int32_t mode = 0;
             const int est = wf_encode_job(j, 0, wf_perm4_enabled(), sa, sa + round16(n1), &mode);
+            // A job's no-maximum flag, as in a plan (the kernel has the
+            // instances without the running maximum).
+            if (est == WF_OK && j.problem == WF_SW && !(mode & WFK_MODE_CMP) && (j.flags & WF_TILE_NO_MAX))
+                mode |= WFK_MODE_NOMAX;
             const int32_t slot = r->free_slots.back();
             if (est != WF_OK) {
                 r->codes_alloc.release(q.codes_off, q.codes_bytes);
@@ -627,6 +633,7 @@ static void resident_scheduler(wf_server *s)
             pd.n2 = (int32_t)n2;
             pd.prob = q.req.problem;
             pd.mode = mode;
+            q.nomax = (mode & WFK_MODE_NOMAX) != 0;
             // Longer dependency chains (columns) issue first, as in a plan
             // launch, so that a long pair's strips are not starved by the
             // short ones that share their SIMDs.

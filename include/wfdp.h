@@ -98,9 +98,8 @@ typedef struct {
                                    GPU_Hyperlane_Share.cu:781), not the running maximum
                                    (about 8% of the cell engines' instructions); such pairs
                                    report max_score = -1.  Pairs with more symbols than the
-                                   kernels' row tables hold, and requests run by the
-                                   resident server's kernel, keep the maximum.  Also a job
-                                   flag (wf_job.flags). */
+                                   kernels' row tables hold keep the maximum.  Also a job
+                                   flag (wf_job.flags), which the server honours. */
 
 typedef struct {
     int32_t last;      /* bottom-right cell: the score the reference prints */

@@ -192,8 +192,7 @@ def test_resident_server_empty_and_textbook_and_large_alphabets():
 @pytest.mark.parametrize("resident", MODES)
 def test_server_no_max_requests(resident):
     """SW requests with the no_max job flag: the bottom-right cell equals the
-    oracle's; launches on stream slots skip the maximum (max_score -1), the
-    resident kernel computes it anyway."""
+    oracle's and the maximum is skipped (max_score -1) in both modes."""
     rng = np.random.default_rng(34)
     jobs = [Job("SW", rng.integers(0, 4, int(rng.integers(100, 2500)), dtype=np.int32),
                 rng.integers(0, 4, int(rng.integers(100, 2500)), dtype=np.int32), no_max=True) for _ in range(16)]
@@ -204,4 +203,4 @@ def test_server_no_max_requests(resident):
     for t, j in zip(tickets, jobs):
         last, mx = O.score("SW", j.a, j.b)
         assert got[t].status == 0 and got[t].last == last
-        assert got[t].max_score == (mx if resident else -1)
+        assert got[t].max_score == -1
