@@ -614,6 +614,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         uint64_t t;
         uint32_t c;
         int base;
+        int width;  // columns polled: lanes past it count as not ready
 #ifdef WFK_TRACE
         int64_t t_issue;
 #endif
@@ -622,8 +623,14 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
     int64_t tr_lat[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tr_cols[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int tr_polls[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #endif
+    // Poll width: 32 columns while the polls keep up with the producing
+    // strip (a poll then finds about 20 new columns), 64 after a poll that
+    // came back ready in full (catching up); narrower polls fetch fewer
+    // boundary words that are not written yet.
+    int width = 64;
     auto issue = [&](Poll &p) __attribute__((always_inline)) {
         p.base = in_next;
+        p.width = width;
 #ifdef WFK_TRACE
         p.t_issue = __builtin_amdgcn_s_memrealtime();
 #endif
@@ -631,7 +638,9 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         const bool in = x < n1;
         p.c = in ? (uint32_t)acodes[x] : 0u;
         if (P == WFK_SWP) p.c = Cell<WFK_SWP, false>::selector(p.c & 3u);
-        p.t = (has_prev && in) ? __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tag;
+        p.t = (has_prev && in)
+                  ? (lane < p.width ? __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0)
+                  : tag;
     };
     // Stage what poll p shows beyond in_next, within the room the compute
     // waves have freed: ring[0] slot of column c is free once wave 0 has
@@ -641,6 +650,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         const bool stale = (p.t >> 32) != (uint64_t)epoch;
         const uint64_t smask = __builtin_amdgcn_ballot_w64(stale);
         const int ready = smask ? (int)__builtin_ctzll(smask) : 64;
+        width = (ready >= p.width) ? 64 : 32;
         const int room = min(p0 + RS, pl - 63 - G + AW);
         const int limit = min(min(n1, p.base + ready), room);
 #ifdef WFK_TRACE
