@@ -2,8 +2,8 @@
 answers stored by tests/golden/make_fullsize.py (the oracle needs about a
 minute of CPU for them), and through size-independent properties -- the
 LCS and SW scores are symmetric in the two sequences, textbook edit
-distance too, both engines agree, and every result of the served trace
-equals the per-pair launch of the same requests."""
+distance too, both engines agree -- and every result of the full served
+trace, in both serving modes, equals the oracle's."""
 import hashlib
 import json
 from pathlib import Path
@@ -104,17 +104,25 @@ def test_sw_batch_4096(ctx):
 
 
 @pytest.mark.gpu
-def test_served_trace_equals_per_pair(ctx):
-    """The full 600-request served mix (lengths up to 131072): the
-    continuous-batching server returns what per-pair launches return."""
+def test_served_trace_equals_oracle(ctx):
+    """The full 600-request served mix (lengths up to 131072, 7.6e10 cells)
+    in both serving modes, replayed at 50 000 requests/s: every result
+    equals the oracle's (its all-cores variant, anti-diagonals of 256 x 256
+    tiles, a few seconds on 16 host threads) and the per-pair launch's."""
+    import os
+    import oracle as O
     from gpumemefficient_amd.server import Server
     reqs = G.serving_requests(50000.0, 600, seed=5)
     jobs = [Job(r.problem, r.a, r.b) for r in reqs]
-    with Server(0) as srv:
-        comps = srv.replay(jobs, [r.arrival_s for r in reqs])
+    threads = max(1, min(16, os.cpu_count() or 1))
+    want = [O.blocked(j.problem, j.a, j.b, threads, 256) for j in jobs]
+    for resident in (False, True):
+        with Server(0, resident=resident) as srv:
+            comps = srv.replay(jobs, [r.arrival_s for r in reqs])
+        print(f"  resident={resident}: {len(comps)} completions", flush=True)
+        assert all(c.status == 0 for c in comps)
+        assert [(c.last, c.max_score) for c in comps] == want
     plan = ctx.plan(jobs)
     plan.run(per_pair=True)
-    want = [(r.last, r.max_score) for r in plan.results()]
+    assert [(r.last, r.max_score) for r in plan.results()] == want
     plan.close()
-    assert all(c.status == 0 for c in comps)
-    assert [(c.last, c.max_score) for c in comps] == want
