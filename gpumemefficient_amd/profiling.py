@@ -73,12 +73,17 @@ def counters(pmc_dir: str):
     return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
 
 
-def summarize(trace: str | None, pmcs: list[str], out: str, title: str, command: str) -> None:
+def summarize(traces: list[str], pmcs: list[str], out: str, title: str, command: str, note: str = "") -> None:
+    """Markdown summary of kernel traces and counter runs.  A trace given as
+    LABEL=DIR gets a section titled with its label."""
     lines = [f"# {title}", "", "Command:", "", "```", command, "```", ""]
-    if trace:
-        lines += ["## Kernel trace (rocprofv3 --kernel-trace --stats)", "",
-                  "| kernel | calls | total ms | avg us | min us | max us |", "|---|---|---|---|---|---|"]
-        for k, n, tot, avg, mn, mx in kernel_stats(trace):
+    if note:
+        lines += [note, ""]
+    for t in traces:
+        label, _, d = t.rpartition("=")
+        head = f"## Kernel trace: {label}" if label else "## Kernel trace (rocprofv3 --kernel-trace --stats)"
+        lines += [head, "", "| kernel | calls | total ms | avg us | min us | max us |", "|---|---|---|---|---|---|"]
+        for k, n, tot, avg, mn, mx in kernel_stats(d):
             lines.append(f"| `{k[:90]}` | {n} | {tot / 1e6:.3f} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} |")
         lines.append("")
     for d in pmcs:
@@ -195,11 +200,12 @@ def main(argv=None) -> int:
     c = sub.add_parser("copy")
     c.add_argument("--bytes", type=int, default=1 << 30)
     s = sub.add_parser("summarize")
-    s.add_argument("--trace")
+    s.add_argument("--trace", action="append", default=[], help="DIR or LABEL=DIR; may repeat")
     s.add_argument("--pmc", action="append", default=[])
     s.add_argument("--out", required=True)
     s.add_argument("--title", default="rocprofv3 summary")
     s.add_argument("--command", default="")
+    s.add_argument("--note", default="", help="a paragraph placed after the command")
     m = sub.add_parser("metrics")
     m.add_argument("--trace", required=True)
     m.add_argument("--pmc", action="append", default=[])
@@ -224,7 +230,7 @@ def main(argv=None) -> int:
         metrics(a.trace, a.pmc, a.kernel, a.problem, a.n1, a.n2, a.strip_rows, a.cus, a.clock_mhz, ref, a.title,
                 a.command, a.out)
     else:
-        summarize(a.trace, a.pmc, a.out, a.title, a.command)
+        summarize(a.trace, a.pmc, a.out, a.title, a.command, a.note)
     return 0
 
 
