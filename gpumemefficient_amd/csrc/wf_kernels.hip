@@ -50,9 +50,10 @@ struct WgShared {
     // ring[0]: this strip's top input (lane 0 of wave 0); ring[w+1]: bottom
     // lane of wave w, read by wave w+1 (and by the copy-out for w = W-1).
     __attribute__((aligned(16))) int ring[W + 1][RS];
-    // Lanes 0..62 store their bottom cell here so that the capture store is
-    // one unmasked ds_write per step; only lane 63's store lands in a ring.
-    int dummy[W][64 + K];
+    // Lanes 0..62 store their bottom cells here (lane l at 4l plus the
+    // group's offset) so that a 4-step group's capture is one unmasked
+    // ds_write_b128; only lane 63's store lands in a ring.
+    __attribute__((aligned(16))) int dummy[W][64 * 4 + K];
     // Window of column codes, one per dword (aligned dword reads are the
     // cheapest LDS reads for the lane-skewed access): column x lives at
     // x mod AW, the first K slots are mirrored after the end so a chunk never
@@ -79,13 +80,17 @@ struct DumpRows {
 template <int P, bool CMP, int R, int K, int AW, bool MASKED, bool TRACK, bool DUMP = false>
 __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, int n1,
                                           const int *ring_prev, int *ring_cur, int *dummy,
-                                          const uint32_t *aw, DumpRows dr = DumpRows{})
+                                          const uint32_t *aw, int &pend, DumpRows dr = DumpRows{})
 {
     constexpr int RS = ChunkGeom<K>::RS;
     const int *__restrict__ inj_base = ring_prev + ((s0 + 64) & (RS - 1));
     const uint32_t *__restrict__ cp = aw + ((s0 - lane) & (AW - 1));
-    int *cap = (lane == 63) ? ring_cur + ((s0 + 1) & (RS - 1)) : dummy + lane;
-    int *cap_last = (lane == 63) ? ring_cur + ((s0 + K) & (RS - 1)) : dummy + lane + (K - 1);
+    // Lane 63's bottom cell of step s belongs in ring slot s + 1.  After the
+    // 4-step group at S the values of steps S-1..S+2 go to slots S..S+3 in
+    // one 16-byte store; step S+3's value is carried in `pend` and stored
+    // alone at the chunk's end, before the barrier that hands the chunk to
+    // the wave below.
+    int *capb = (lane == 63) ? ring_cur + (s0 & (RS - 1)) : dummy + lane * 4;
     // Software pipeline: the LDS reads of group q+1 (4 column codes and the
     // 4 top-neighbour values of lane 0) are issued before group q's capture
     // stores, so their latency hides behind group q's arithmetic.
@@ -100,16 +105,16 @@ __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, in
 #pragma unroll
             for (int u = 0; u < 4; ++u) ca_n[u] = cp[q + 4 + u];
         }
+        // The next group's reads stay here, a group ahead of their use.
+        __builtin_amdgcn_sched_barrier(0x207);
+        int capv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = q + u;
             bool valid = true;
             if (MASKED) valid = (unsigned)(s0 + j - lane) < (unsigned)n1;
             step<P, CMP, R, MASKED, TRACK>(st, injv[u], cav[u], valid);
-            if (j == K - 1)
-                *cap_last = st.h[R - 1];
-            else
-                cap[j] = st.h[R - 1];
+            capv[u] = st.h[R - 1];
             if (DUMP) {
                 const int x = s0 + j - lane;
                 if ((unsigned)x < (unsigned)n1) {
@@ -122,7 +127,10 @@ __device__ __forceinline__ void run_chunk(LaneState<R> &st, int s0, int lane, in
                 }
             }
         }
+        *reinterpret_cast<int4 *>(capb + q) = make_int4(pend, capv[0], capv[1], capv[2]);
+        pend = capv[3];
     }
+    *((lane == 63) ? ring_cur + ((s0 + K) & (RS - 1)) : dummy + lane * 4) = pend;
 }
 
 // One strip (`g`) of one pair, swept by the whole workgroup.
@@ -232,6 +240,7 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
     };
 
     bool aborted = sh.abort_at <= -1;
+    int pend = 0;  // lane 63's last capture not yet in the ring (see run_chunk)
     for (int j = 0; j < total && !aborted; ++j) {
         const int cj = j - wave * D;
         // Wave 0 prefetches the next chunk's column codes and top boundary.
@@ -248,10 +257,10 @@ __device__ void run_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair,
             const DumpRows dr{ka.table, row0, n2};
             if (steady)
                 run_chunk<P, CMP, R, K, AW, false, TRACK, DUMP>(st, s0, lane, n1, sh.ring[wave],
-                                                                sh.ring[wave + 1], sh.dummy[wave], sh.aw, dr);
+                                                                sh.ring[wave + 1], sh.dummy[wave], sh.aw, pend, dr);
             else
                 run_chunk<P, CMP, R, K, AW, true, TRACK, DUMP>(st, s0, lane, n1, sh.ring[wave],
-                                                               sh.ring[wave + 1], sh.dummy[wave], sh.aw, dr);
+                                                               sh.ring[wave + 1], sh.dummy[wave], sh.aw, pend, dr);
             if (wave == W - 1 && has_next) {
                 // Steps < s0 + K are done: lane 63 has finished columns < s0 + K - 63.
                 copy_out(min(n1, s0 + K - 63), true);
