@@ -94,11 +94,19 @@ struct Stage {
 // (rows below the table) read zeros, so every block load is issued
 // unconditionally, with no per-lane branch (see the chunk loop).  Offsets
 // are 32-bit: 73 rows of the pitch must fit (checked by the host).
+// The descriptor's fields pass through readfirstlane: a descriptor the
+// compiler cannot prove uniform lives in VGPRs, and every buffer load then
+// becomes a waterfall loop (readfirstlane, compare, exec mask, load, repeat).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t window_rsrc(const WfStencilArgs &a, int yw)
 {
     const int rows = max(0, min(kTR, a.n2 + 2 - yw));
+    const uint64_t base = reinterpret_cast<uint64_t>(a.t + (int64_t)yw * a.ld);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+    int *p = reinterpret_cast<int *>(((uint64_t)hi << 32) | lo);
+    const int bytes = __builtin_amdgcn_readfirstlane((int)(rows * a.ld * 4));
     // Third descriptor dword of an untyped buffer on the gfx9 family.
-    return __builtin_amdgcn_make_buffer_rsrc(a.t + (int64_t)yw * a.ld, 0, (int)(rows * a.ld * 4), 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
 }
 
 __device__ __forceinline__ void stage_load(Stage &sg, __amdgpu_buffer_rsrc_t rs, int64_t ld, int b, int lane)
