@@ -135,22 +135,46 @@ __device__ __forceinline__ void stage_store(const Stage &sg, int *win, int b, in
     }
 }
 
-// Store block b of the wave's interior rows (r < nrows) back to the table;
-// vectors that hold no interior column are skipped (boundary columns inside
-// a stored vector are written back unchanged).
-__device__ __forceinline__ void flush_block(const WfStencilArgs &a, const int *win, int yw, int b, int nrows,
-                                            int lane)
+// Store block b of the wave's interior rows (r < nrows) back to the table
+// through a buffer descriptor over those rows: a vector outside them, or
+// holding no interior column, gets an offset past the descriptor's end and
+// the hardware drops it (boundary columns inside a stored vector are written
+// back unchanged).  With no branch around the stores the eight LDS reads are
+// issued together, not one round trip per vector.
+__device__ __forceinline__ void flush_block(const WfStencilArgs &a, const int *win, int b, int nrows, int lane,
+                                            __amdgpu_buffer_rsrc_t out)
 {
     if (b < 0 || 32 * b > a.n1) return;
+    int4 val[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v = lane + 64 * k;
+        val[k] = *reinterpret_cast<const int4 *>(win + (v >> 3) * kTS + ((32 * b + 4 * (v & 7)) & (kTW - 1)));
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int v = lane + 64 * k;
         const int rr = v >> 3;
         const int c = 32 * b + 4 * (v & 7);
-        const int4 val = *reinterpret_cast<const int4 *>(win + rr * kTS + (c & (kTW - 1)));
-        if (rr < nrows && c + 3 >= 1 && c <= a.n1)
-            *reinterpret_cast<int4 *>(a.t + (int64_t)(yw + rr) * a.ld + c) = val;
+        const uint32_t off = (rr < nrows && c + 3 >= 1 && c <= a.n1) ? (uint32_t)((rr * a.ld + c) * 4) : 0xfffffff0u;
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) uint32_t, val[k]),
+                                               out, off, 0, 0);
     }
+}
+
+// The wave's interior rows (nrows of them from yw) as the descriptor of
+// flush_block, its fields through readfirstlane as in window_rsrc.  The 4
+// bytes past the last row cover the one dword a stored vector can reach
+// beyond its row when the pitch is not a multiple of 4: the next row's
+// column 0, a boundary value written back unchanged.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t interior_rsrc(const WfStencilArgs &a, int yw, int nrows)
+{
+    const uint64_t base = reinterpret_cast<uint64_t>(a.t + (int64_t)yw * a.ld);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+    int *p = reinterpret_cast<int *>(((uint64_t)hi << 32) | lo);
+    const int bytes = __builtin_amdgcn_readfirstlane(nrows > 0 ? (int)(nrows * a.ld * 4 + 4) : 0);
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
 }
 
 // One chunk of kK steps from s0 for one wave.  `h` is the lane's last updated
@@ -261,6 +285,7 @@ __device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, in
     int *win = sh.win[wave];
     int *wrow = win + lane * kTS;
     const __amdgpu_buffer_rsrc_t rs = window_rsrc(a, yw);
+    const __amdgpu_buffer_rsrc_t outr = interior_rsrc(a, yw, nrows);
 
     // Top row (columns [32c, 32c+32), wave 0, lanes < 32): row 0 of the table
     // for the first strip, else the previous strip's bottom row.
@@ -359,7 +384,7 @@ __device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, in
         // Steps < s0 + kK are done: lane 63 has finished columns < s0 + kK - 62,
         // so every lane has finished block cj-2.
         if (wave == W - 1 && has_next) copy_out(min(n1 + 1, s0 + kK - 62), true);
-        flush_block(a, win, yw, cj - 2, nrows, lane);
+        flush_block(a, win, cj - 2, nrows, lane, outr);
         if (wave == 0 && cj + 2 < nfill) {
             if (!fill(cj + 2, top_old) && lane == 0) sh.abort_at = j;
         }
@@ -390,7 +415,7 @@ __device__ void stencil_strip(const WfStencilArgs &a, int g, StShared<W> &sh, in
     // Blocks not yet flushed: nch-2 .. the one holding column n1 (all of
     // them are still in the window).
     if (active)
-        for (int b = max(0, nch - 2); 32 * b <= n1; ++b) flush_block(a, win, yw, b, nrows, lane);
+        for (int b = max(0, nch - 2); 32 * b <= n1; ++b) flush_block(a, win, b, nrows, lane, outr);
 }
 
 template <int OP, int W>
