@@ -126,11 +126,22 @@ __device__ __forceinline__ void stage_store(const Stage &sg, int *win, int b, in
 #pragma unroll
     for (int k = 0; k < (kVec + 63) / 64; ++k) {
         const int v = lane + 64 * k;
-        if (v < kVec) {
-            const int rr = v >> 3;
+        if (k < kVec / 64 || v < kVec) {
             const int slot = (32 * b + 4 * (v & 7)) & (kTW - 1);
-            *reinterpret_cast<int4 *>(win + rr * kTS + slot) = sg.v[k];
-            if (slot == 0) *reinterpret_cast<int4 *>(win + rr * kTS + kTW) = sg.v[k];
+            *reinterpret_cast<int4 *>(win + (v >> 3) * kTS + slot) = sg.v[k];
+        }
+    }
+    // A block at slot 0 is mirrored at slots 128..131 (the vectors of lanes
+    // with v % 8 == 0); the other lanes rewrite their own vector, so the
+    // stores need no per-lane branch.
+    if ((b & 3) == 0) {
+#pragma unroll
+        for (int k = 0; k < (kVec + 63) / 64; ++k) {
+            const int v = lane + 64 * k;
+            if (k < kVec / 64 || v < kVec) {
+                const int slot = (v & 7) == 0 ? kTW : 4 * (v & 7);
+                *reinterpret_cast<int4 *>(win + (v >> 3) * kTS + slot) = sg.v[k];
+            }
         }
     }
 }
