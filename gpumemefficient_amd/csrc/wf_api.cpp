@@ -371,6 +371,9 @@ static bool split_enabled()
 // unless WFDP_PERM=0 (wf_perm4_enabled).
 // Pairs narrower than this are not split.
 constexpr int64_t kSplitMinCols = 1024;
+// SW pairs narrower than this are not split (three sweeps of half the
+// columns only pay where the strip chain's fill is short against them).
+constexpr int64_t kSwSplitMinCols = 4096;
 
 int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
                   WfPlanHost *ph)
@@ -397,7 +400,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     // pair's time is its column count times the step time, so the halves
     // finish in about half of it.
     std::vector<int64_t> mid(njobs, 0);
-    int64_t nsplit = 0;
+    std::vector<uint8_t> swsplit(njobs, 0);
+    int64_t nsplit = 0, nextra = 0;
     if (bitpar && split_enabled()) {
         int64_t strips = 0;
         for (int64_t i = 0; i < njobs; ++i)
@@ -409,8 +413,41 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
                     mid[i] = jobs[i].n1 / 2;
                     ++nsplit;
                 }
+        nextra = nsplit;
+    } else if (v->engine == WF_ENGINE_ASYNC && perm4 && split_enabled()) {
+        // SW pairs whose output is the bottom-right cell only, on the byte
+        // tables, split in three (WfSplitDev) when all of the launch's
+        // strips fit on the chip at once: the halves' strip chains run side
+        // by side in half the columns.
+        int64_t strips = 0, extra = 0;
+        std::vector<uint8_t> cand(njobs, 0);
+        for (int64_t i = 0; i < njobs; ++i) {
+            const wf_job &j = jobs[i];
+            if (j.n1 <= 0 || j.n2 <= 0) continue;
+            const int64_t si = (j.n2 + strip - 1) / strip;
+            strips += si;
+            if (j.problem != WF_SW || j.n1 < kSwSplitMinCols ||
+                !((j.flags | (tile ? tile->flags : 0)) & WF_TILE_NO_MAX))
+                continue;
+            std::vector<uint8_t> ta((size_t)j.n1), tb((size_t)j.n2);
+            int32_t mode = 0;
+            if (wf_encode_job(j, tile ? tile->flags : 0, perm4, ta.data(), tb.data(), &mode) != WF_OK ||
+                !(mode & WFK_MODE_PERM4))
+                continue;
+            cand[i] = 1;
+            extra += 2 * si;
+        }
+        const int64_t room = ctx ? wf_max_resident_groups(ctx, v) : 256;
+        if (extra > 0 && strips + extra <= room)
+            for (int64_t i = 0; i < njobs; ++i)
+                if (cand[i]) {
+                    mid[i] = jobs[i].n1 / 2;
+                    swsplit[i] = 1;
+                    ++nsplit;
+                    nextra += 2;
+                }
     }
-    const int64_t np = njobs + nsplit;
+    const int64_t np = njobs + nextra;
     ph->njobs = njobs;
     ph->npairs = np;
     ph->pairs.assign(np, WfPairDev{});
@@ -459,6 +496,30 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         for (int64_t i = 0; i < njobs; ++i) {
             if (!mid[i]) continue;
             const wf_job &j = jobs[i];
+            if (swsplit[i]) {
+                // Right half from a zero boundary (pair r), reversed right
+                // half (pair r + 1); each half keeps its last column.
+                const int64_t nr = j.n1 - mid[i];
+                place(ph->pairs[r], j.problem, nr, j.n2, nr);
+                place(ph->pairs[r + 1], j.problem, nr, j.n2, nr);
+                WfSplitDev sp{};
+                sp.job = (int32_t)i;
+                sp.rev = (int32_t)(r + 1);
+                sp.prob = WFK_SW;
+                sp.n2 = (int32_t)j.n2;
+                sp.f0 = (int32_t)r;
+                sp.r0 = (int32_t)nr;
+                const int64_t entries = (j.n2 + 1) / 2;  // two words per entry, one per row
+                ph->pairs[i].col_off = ph->colw_entries;
+                sp.fwd_off = ph->colw_entries;
+                ph->colw_entries += entries;
+                ph->pairs[r + 1].col_off = ph->colw_entries;
+                sp.rev_off = ph->colw_entries;
+                ph->colw_entries += entries;
+                ph->splits.push_back(sp);
+                r += 2;
+                continue;
+            }
             place(ph->pairs[r], j.problem, j.n1 - mid[i], j.n2, j.n1 - mid[i]);
             WfSplitDev sp{};
             sp.job = (int32_t)i;
@@ -492,6 +553,27 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         if (bitpar && (j.problem == WF_SW || (mode & WFK_MODE_CMP))) return WF_ERR_NOKERNEL;
     }
     for (WfSplitDev &sp : ph->splits) {
+        if (sp.prob == WFK_SW) {
+            // Right half: a[mid..n1) and b as they are, the pair's modes;
+            // reverse half: both reversed, the unclamped recurrence.
+            const wf_job &j = jobs[sp.job];
+            const WfPairDev &f = ph->pairs[sp.job];
+            WfPairDev &rp = ph->pairs[sp.f0];
+            WfPairDev &rv = ph->pairs[sp.rev];
+            const int64_t m = mid[sp.job], nr = sp.r0;
+            const uint8_t *fa = ph->codes.data() + f.a_off;
+            const uint8_t *fb = ph->codes.data() + f.b_off;
+            std::memcpy(ph->codes.data() + rp.a_off, fa + m, (size_t)nr);
+            std::memcpy(ph->codes.data() + rp.b_off, fb, (size_t)j.n2);
+            uint8_t *ra = ph->codes.data() + rv.a_off;
+            uint8_t *rb = ph->codes.data() + rv.b_off;
+            for (int64_t k = 0; k < nr; ++k) ra[k] = fa[j.n1 - 1 - k];
+            for (int64_t k = 0; k < j.n2; ++k) rb[k] = fb[j.n2 - 1 - k];
+            rp.mode = f.mode;
+            rv.mode = (f.mode & (WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD)) | WFK_MODE_NOMAX | WFK_MODE_SWE;
+            if (!(f.mode & WFK_MODE_PERM4) || !(f.mode & WFK_MODE_NOMAX)) return WF_ERR_INVALID;
+            continue;
+        }
         // Reverse half: a[mid..n1) and b reversed, textbook ED boundary (its
         // row 0 is a plain distance from the empty prefix).
         const wf_job &j = jobs[sp.job];

@@ -41,6 +41,7 @@
 // match masks for the compute waves) and the combine kernel of pairs split
 // into a forward and a reversed column half (WfSplitDev).
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <stdint.h>
 
 #include "wf_cells.h"
@@ -718,10 +719,47 @@ __device__ void bs_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, 
 // halves line up row by row.
 // ---------------------------------------------------------------------------
 
+// Column-split SW pair (prob WFK_SW): job holds columns 1..mid, f0 the
+// columns mid+1..n1 from a zero boundary (its bottom-right cell R0: the best
+// local alignment starting right of the cut), rev the reversed right half,
+// whose last column E(i) gives the best path weight from (n2 - i, mid) to
+// (n2, n1) as E(i) - 2 (i + r0).  The bottom-right cell is
+// max(R0, max_y H(y, mid) + that weight), H(0, mid) = 0, E(0) = 0.
+__device__ void sw_split_combine(const WfSplitDev &sp, const uint32_t *colw, WfResultDev *res)
+{
+    const int n2 = sp.n2, nr = sp.r0;
+    const int *h = reinterpret_cast<const int *>(colw) + 2 * sp.fwd_off;
+    const int *e = reinterpret_cast<const int *>(colw) + 2 * sp.rev_off;
+    int best = INT_MIN;
+    for (int y = threadIdx.x; y <= n2; y += blockDim.x) {
+        const int i = n2 - y;
+        const int hv = y > 0 ? h[y - 1] : 0;
+        const int ev = i > 0 ? e[i - 1] : 0;
+        best = max(best, hv + ev - 2 * (i + nr));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off));
+    __shared__ int wb[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) wb[wv] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wb[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) b = max(b, wb[k]);
+        WfResultDev &r = res[sp.job];
+        r.last = max(b, res[sp.f0].last);
+        r.status |= res[sp.rev].status | res[sp.f0].status;
+    }
+}
+
 __global__ __launch_bounds__(1024) void wf_bitpar_combine(const WfSplitDev *splits, const uint32_t *colw,
                                                           WfResultDev *res)
 {
     const WfSplitDev sp = splits[blockIdx.x];
+    if (sp.prob == WFK_SW) {
+        sw_split_combine(sp, colw, res);
+        return;
+    }
     const bool ed = sp.prob == WFK_ED;
     const int n2 = sp.n2;
     const int nw = (n2 + 31) >> 5;

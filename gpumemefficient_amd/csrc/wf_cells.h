@@ -126,6 +126,28 @@ struct Cell<WFK_SWP, CMP> {
     static constexpr uint32_t kNever = 0x0c0c0d0du;
 };
 
+// The reverse half of a column-split SW pair (WFK_MODE_SWE): the same byte
+// tables and selectors with s + 4 (7 on a match, 1 else) and no clamp,
+// E = max3(up, left, diag + s + 4).  A never-matching column reads -1, so
+// lanes left of column 0 keep the zero boundary.
+template <bool CMP>
+struct Cell<WFK_SWE, CMP> {
+    static constexpr int idle = 0;
+    static __device__ __forceinline__ uint32_t tables(uint32_t c0, uint32_t c1, int half)
+    {
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t sym = 2u * (uint32_t)half + (uint32_t)k;
+            t |= (c0 == sym ? 0x07u : 0x01u) << (16 * k);
+            t |= (c1 == sym ? 0x07u : 0x01u) << (16 * k + 8);
+        }
+        return t;
+    }
+    static __device__ __forceinline__ uint32_t selector(uint32_t c) { return Cell<WFK_SWP, CMP>::selector(c); }
+    static constexpr uint32_t kNever = Cell<WFK_SWP, CMP>::kNever;
+};
+
 template <int R>
 struct LaneState {
     int h[R];        // H(row r, last column) -- the next step's "left"
@@ -141,16 +163,16 @@ __device__ __forceinline__ void step(LaneState<R> &st, int inj, uint32_t ca, boo
     int up = up0;
     int diag = st.dg;
     uint32_t pp = 0;
-    if (P == WFK_SWP) pp = __builtin_amdgcn_perm(st.t[1], st.t[0], ca);
+    if (P == WFK_SWP || P == WFK_SWE) pp = __builtin_amdgcn_perm(st.t[1], st.t[0], ca);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int left = st.h[r];
         int h;
-        if constexpr (P == WFK_SWP) {
-            static_assert(R == 2, "WFK_SWP holds two rows per lane");
+        if constexpr (P == WFK_SWP || P == WFK_SWE) {
+            static_assert(R == 2, "the byte tables hold two rows per lane");
             const int s2 = (int)(int8_t)(uint8_t)(pp >> (8 * r));
             const int m = max(max(up, left), diag + s2);
-            h = (int)__builtin_elementwise_sub_sat((uint32_t)m, 2u);
+            h = (P == WFK_SWE) ? m : (int)__builtin_elementwise_sub_sat((uint32_t)m, 2u);
         } else {
             h = Cell<P, CMP>::f(up, left, diag, ca, st.t[r]);
         }

@@ -6,8 +6,10 @@
 
 // Problem ids match include/wfdp.h (WF_LCS, WF_ED, WF_SW).
 enum { WFK_LCS = 0, WFK_ED = 1, WFK_SW = 2 };
-// Kernel-internal problem id: SW in WFK_MODE_PERM4 (never stored in a pair).
-enum { WFK_SWP = 3 };
+// Kernel-internal problem ids (never stored in a pair): SW in
+// WFK_MODE_PERM4, and the reverse half of a column-split SW pair
+// (WFK_MODE_SWE).
+enum { WFK_SWP = 3, WFK_SWE = 4 };
 
 // Pair mode bits.
 enum {
@@ -24,6 +26,11 @@ enum {
     WFK_MODE_PERM4 = 16,
     // SW without the running maximum (only the bottom-right cell).
     WFK_MODE_NOMAX = 32,
+    // The reverse half of a column-split SW pair (with WFK_MODE_PERM4): the
+    // unclamped best path weight into each cell of the reversed sequences,
+    // kept as E = F + 2 (x + y), so E = max3(up, left, diag + s + 4) with
+    // zero boundaries.
+    WFK_MODE_SWE = 64,
 };
 enum { WFK_MODE_NEVER_SHIFT = 8 };
 
@@ -56,13 +63,18 @@ struct WfPairDev {
 // half's column mid and R(y) the reverse half's last column at row n2 - y,
 // the score is min (ED) / max (LCS) over y of F(y) + R(y) (Hirschberg's
 // split, J. ACM 1975; CACM 18(6) 1975 for LCS).
+//
+// An SW pair (prob WFK_SW, cell engine, bottom-right cell only) splits into
+// three: `job` (columns 1..mid), `f0` (columns mid+1..n1 from a zero
+// boundary) and `rev` (the reversed right half, WFK_MODE_SWE); r0 is the
+// right half's column count (see sw_split_combine in wf_bitpar.hip).
 struct WfSplitDev {
     int32_t job;       // result slot (the forward half's pair)
     int32_t rev;       // the reverse half's pair
-    int32_t prob;      // WFK_ED / WFK_LCS
+    int32_t prob;      // WFK_ED / WFK_LCS / WFK_SW
     int32_t n2;        // rows
-    int32_t f0;        // ED: F(0), row 0 of the table at column mid; LCS: 0
-    int32_t r0;        // ED: the reverse half's row 0 at its last column (n1 - mid); LCS: 0
+    int32_t f0;        // ED: F(0), row 0 of the table at column mid; LCS: 0; SW: the right half's pair
+    int32_t r0;        // ED: the reverse half's row 0 at its last column (n1 - mid); LCS: 0; SW: n1 - mid
     int64_t fwd_off;   // colw entry offsets of the two halves
     int64_t rev_off;
 };

@@ -61,7 +61,7 @@ constexpr int kUMax = 256;
 template <int P, int W, int G>
 constexpr int steady_unroll()
 {
-    return (P == WFK_SWP && W == 4 && G == 32) ? kUMax : kU;
+    return ((P == WFK_SWP || P == WFK_SWE) && W == 4 && G == 32) ? kUMax : kU;
 }
 constexpr int kB = 8;  // a 4-step block (prefetch 4 steps ahead) measured slower: 7.76 against 7.71 ms
 
@@ -360,7 +360,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         const int64_t row0 = (int64_t)g * (W * 64 * R) + w * (64 * R) + lane * R;
         const uint8_t *bcodes = ka.codes + pd.b_off;
         LaneState<R> st;
-        if constexpr (P == WFK_SWP) {
+        if constexpr (P == WFK_SWP || P == WFK_SWE) {
             const uint32_t c0 = row0 < n2 ? (uint32_t)bcodes[row0] : 4u;
             const uint32_t c1 = row0 + 1 < n2 ? (uint32_t)bcodes[row0 + 1] : 4u;
             st.t[0] = Cell<P, CMP>::tables(c0, c1, 0);
@@ -540,6 +540,13 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
                 for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
                 if (lane == 0 && m > 0) atomicMax(&ka.res[pair].maxs, m);
             }
+            // The halves of a column-split SW pair keep their last column
+            // (row y at word 2 * col_off + y) for the combine kernel.
+            if ((P == WFK_SWP || P == WFK_SWE) && pd.col_off >= 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (row0 + r < n2) ka.colw[2 * pd.col_off + row0 + r] = (uint32_t)st.h[r];
+            }
         }
 #ifdef WFK_TRACE
         tr_store();
@@ -625,7 +632,8 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
     // counter store, so every compute wave reads them after they are set.
     {
         const uint32_t never =
-            (P == WFK_SWP) ? Cell<WFK_SWP, false>::kNever : ((uint32_t)pd.mode >> WFK_MODE_NEVER_SHIFT) & 0x1FFu;
+            (P == WFK_SWP || P == WFK_SWE) ? Cell<WFK_SWP, false>::kNever
+                                           : ((uint32_t)pd.mode >> WFK_MODE_NEVER_SHIFT) & 0x1FFu;
         sh.aw[lane] = never;
         if (lane < kUMax + kB) sh.aw[AW + lane] = never;
     }
@@ -657,7 +665,7 @@ __device__ __forceinline__ void async_strip(const WfKernelArgs &ka, const WfPair
         const int x = in_next + lane;
         const bool in = x < n1;
         p.c = in ? (uint32_t)acodes[x] : 0u;
-        if (P == WFK_SWP) p.c = Cell<WFK_SWP, false>::selector(p.c & 3u);
+        if (P == WFK_SWP || P == WFK_SWE) p.c = Cell<WFK_SWP, false>::selector(p.c & 3u);
         p.t = (has_prev && in)
                   ? (lane < p.width ? __hip_atomic_load(bprev + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0)
                   : tag;
@@ -784,6 +792,11 @@ __global__ __launch_bounds__(64 * (W + 2)) void wf_engine_async(WfKernelArgs ka)
             // Instances without the running maximum for WFK_MODE_NOMAX.
             const bool nomax = (pd.mode & WFK_MODE_NOMAX) != 0;
             if constexpr (R == 2) {
+                if (pd.mode & WFK_MODE_SWE) {
+                    async_strip<WFK_SWE, false, R, W, G, AW, RS, true>(ka, pd, td.pair, td.group, sh, lane, wave,
+                                                                      epoch, t);
+                    break;
+                }
                 if (pd.mode & WFK_MODE_PERM4) {
                     if (nomax)
                         async_strip<WFK_SWP, false, R, W, G, AW, RS, true>(ka, pd, td.pair, td.group, sh, lane, wave,
