@@ -115,8 +115,28 @@ def strip_bytes(n1: int, n2: int, strip_rows: int) -> tuple[int, int]:
     return boundary + strips * n1 + n2, boundary
 
 
+def split_widths(n1: int) -> list[int]:
+    """Column counts of the three pairs an SW column split launches: the
+    left columns 1..n1/2, the right columns from a zero boundary and the
+    reversed right columns."""
+    mid = n1 // 2
+    return [mid, n1 - mid, n1 - mid]
+
+
+def split_bytes(n1: int, n2: int, strip_rows: int) -> tuple[int, int]:
+    """strip_bytes of the three pairs of an SW column split, plus the last
+    column (n2 four-byte words) that the left and the reversed pair store
+    for the combine kernel."""
+    load_b = store_b = 0
+    for w in split_widths(n1):
+        lb, sb = strip_bytes(w, n2, strip_rows)
+        load_b, store_b = load_b + lb, store_b + sb
+    return load_b, store_b + 2 * n2 * 4
+
+
 def metrics(trace: str, pmcs: list[str], kernel: str, problem: str, n1: int, n2: int, strip_rows: int,
-            cus: int, clock_mhz: float, ref: dict, title: str, command: str, out: str) -> None:
+            cus: int, clock_mhz: float, ref: dict, title: str, command: str, out: str,
+            column_split: bool = False) -> None:
     """rocprofv3 counterparts of the reference's nvprof metrics for the
     dispatches of `kernel` (mean per dispatch):
       gld_efficiency       algorithmic load bytes / HBM bytes fetched
@@ -126,7 +146,8 @@ def metrics(trace: str, pmcs: list[str], kernel: str, problem: str, n1: int, n2:
       l2_tex_read_hit_rate TCC_HIT / (TCC_HIT + TCC_MISS)
       inst_integer         SQ_INSTS_VALU_INT32 (wave instructions) x 64 lanes
     plus the VALU issue utilisation and the int32 VALU rate against the
-    chip's peak."""
+    chip's peak.  With column_split the dispatch is the three pairs of an
+    SW column split (split_widths): 1.5x the cells of the pair."""
     c = {}
     for d in pmcs:
         for k, cs in counters(d).items():
@@ -136,7 +157,8 @@ def metrics(trace: str, pmcs: list[str], kernel: str, problem: str, n1: int, n2:
     if not stats:
         raise SystemExit(f"no dispatch of {kernel} in {trace}")
     name, calls, _, avg_ns, _, _ = stats[0]
-    load_b, store_b = strip_bytes(n1, n2, strip_rows)
+    load_b, store_b = (split_bytes if column_split else strip_bytes)(n1, n2, strip_rows)
+    cells = n2 * (sum(split_widths(n1)) if column_split else n1)
     fetch = c.get("FETCH_SIZE", 0.0) * 1024 * FETCH_SCALE
     write = c.get("WRITE_SIZE", 0.0) * 1024 * WRITE_SCALE
     peak = cus * 64 * clock_mhz * 1e6
@@ -179,13 +201,14 @@ def metrics(trace: str, pmcs: list[str], kernel: str, problem: str, n1: int, n2:
         extra.append(f"- int32 VALU lane ops per dispatch {int_ops:.4g} over {avg_ns / 1e3:.1f} us = "
                      f"{int_ops / (avg_ns * 1e-9) / 1e12:.2f} TOP/s, {100 * int_ops / (avg_ns * 1e-9) / peak:.1f}% "
                      f"of the {peak / 1e12:.2f} TOP/s int32 VALU peak ({cus} CUs x 64 lanes x {clock_mhz:.0f} MHz)")
-        cells = n1 * n2
         extra.append(f"- counted int32 lane ops per cell: {int_ops / cells:.2f} (algorithmic: "
                      f"{OPS_PER_CELL[problem]}); all VALU instructions per cell: "
                      f"{c.get('SQ_INSTS_VALU', 0.0) * 64 / cells:.2f}")
     if fetch or write:
         extra.append(f"- HBM bytes per dispatch: fetched {fetch:.4g}, written {write:.4g}; algorithmic "
-                     f"{load_b + store_b:.4g} ({n2} rows in strips of {strip_rows}, n1 = {n1})")
+                     f"{load_b + store_b:.4g} ({n2} rows in strips of {strip_rows}, n1 = {n1}"
+                     + (f", column split into pairs of {split_widths(n1)} columns, {cells} cells)"
+                        if column_split else ")"))
     lines += extra + ["", "Raw counters (mean per dispatch):", "", "| counter | value |", "|---|---|"]
     for k in sorted(c):
         lines.append(f"| {k} | {c[k]:.6g} |")
@@ -214,6 +237,8 @@ def main(argv=None) -> int:
     m.add_argument("--n1", type=int, required=True)
     m.add_argument("--n2", type=int, required=True)
     m.add_argument("--strip-rows", type=int, required=True)
+    m.add_argument("--column-split", action="store_true",
+                   help="the dispatch runs an SW pair as three column pairs")
     m.add_argument("--cus", type=int, default=256)
     m.add_argument("--clock-mhz", type=float, default=2400.0)
     for key in ("gld", "gst", "shared", "tex", "l2", "inst-integer"):
@@ -228,7 +253,7 @@ def main(argv=None) -> int:
         ref = {"gld": a.ref_gld, "gst": a.ref_gst, "shared": a.ref_shared, "tex": a.ref_tex, "l2": a.ref_l2,
                "inst_integer": a.ref_inst_integer}
         metrics(a.trace, a.pmc, a.kernel, a.problem, a.n1, a.n2, a.strip_rows, a.cus, a.clock_mhz, ref, a.title,
-                a.command, a.out)
+                a.command, a.out, a.column_split)
     else:
         summarize(a.trace, a.pmc, a.out, a.title, a.command, a.note)
     return 0

@@ -70,3 +70,22 @@ def test_metrics_counterparts(tmp_path):
     assert f"{100 * gld:.2f}%" in text and f"{100 * gst:.2f}%" in text
     assert "| shared_efficiency | - | 1 - SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE | 75.00% |" in text
     assert "| gld_efficiency | 86.30% |" in text
+
+
+def test_metrics_column_split(tmp_path):
+    # An SW pair of 1025 columns splits into pairs of 512, 513 and 513.
+    _trace(tmp_path / "t", [("wf_engine_async<2, 4, 32>", 0, 1_000_000)])
+    _pmc(tmp_path / "c", [("wf_engine_async<2, 4, 32>", "FETCH_SIZE", 20.0),
+                          ("wf_engine_async<2, 4, 32>", "WRITE_SIZE", 10.0),
+                          ("wf_engine_async<2, 4, 32>", "SQ_INSTS_VALU_INT32", 1000)])
+    assert P.split_widths(1025) == [512, 513, 513]
+    load_b, store_b = P.split_bytes(1025, 1024, 512)
+    assert store_b == (512 + 513 + 513) * 8 + 2 * 1024 * 4
+    assert load_b == sum(w * 8 + 2 * w + 1024 for w in (512, 513, 513))
+    out = tmp_path / "m.md"
+    P.main(["metrics", "--trace", str(tmp_path / "t"), "--pmc", str(tmp_path / "c"), "--kernel", "wf_engine_async",
+            "--problem", "SW", "--n1", "1025", "--n2", "1024", "--strip-rows", "512", "--column-split",
+            "--out", str(out)])
+    text = out.read_text()
+    assert f"{1000 * 64 / (1538 * 1024):.2f} (algorithmic: 8)" in text
+    assert "pairs of [512, 513, 513] columns, 1574912 cells" in text
