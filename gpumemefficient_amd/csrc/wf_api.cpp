@@ -415,10 +415,10 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
                 }
         nextra = nsplit;
     } else if (v->engine == WF_ENGINE_ASYNC && perm4 && split_enabled()) {
-        // SW pairs whose output is the bottom-right cell only, on the byte
-        // tables, split in three (WfSplitDev) when all of the launch's
-        // strips fit on the chip at once: the halves' strip chains run side
-        // by side in half the columns.
+        // SW pairs on the byte tables split in three (bottom-right cell
+        // only) or four (with the running maximum) pairs of half the columns
+        // (WfSplitDev) when all of the launch's strips fit on the chip at
+        // once: the halves' strip chains run side by side.
         int64_t strips = 0, extra = 0;
         std::vector<uint8_t> cand(njobs, 0);
         for (int64_t i = 0; i < njobs; ++i) {
@@ -426,25 +426,31 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
             if (j.n1 <= 0 || j.n2 <= 0) continue;
             const int64_t si = (j.n2 + strip - 1) / strip;
             strips += si;
-            if (j.problem != WF_SW || j.n1 < kSwSplitMinCols ||
-                !((j.flags | (tile ? tile->flags : 0)) & WF_TILE_NO_MAX))
-                continue;
+            if (j.problem != WF_SW || j.n1 < kSwSplitMinCols) continue;
+            const bool nomax = ((j.flags | (tile ? tile->flags : 0)) & WF_TILE_NO_MAX) != 0;
             std::vector<uint8_t> ta((size_t)j.n1), tb((size_t)j.n2);
             int32_t mode = 0;
             if (wf_encode_job(j, tile ? tile->flags : 0, perm4, ta.data(), tb.data(), &mode) != WF_OK ||
                 !(mode & WFK_MODE_PERM4))
                 continue;
-            cand[i] = 1;
-            extra += 2 * si;
+            cand[i] = nomax ? 2 : 3;  // extra pairs
+            extra += cand[i] * si;
         }
-        const int64_t room = ctx ? wf_max_resident_groups(ctx, v) : 256;
+        // Four pairs (twice the cells) only pay while every strip has a CU
+        // of its own (SW 65536² with the maximum, 512 strips on 256 CUs:
+        // 3.62 -> 4.27 ms); three still pay at two strips per CU (65536²
+        // without it, 384 strips: 3.18 -> 3.03 ms).
+        const bool four = std::find(cand.begin(), cand.end(), 3) != cand.end();
+        const int64_t cus = ctx ? ctx->prop.multiProcessorCount : 256;
+        const int64_t room = std::min<int64_t>(ctx ? wf_max_resident_groups(ctx, v) : 256,
+                                               four ? cus : INT64_MAX);
         if (extra > 0 && strips + extra <= room)
             for (int64_t i = 0; i < njobs; ++i)
                 if (cand[i]) {
                     mid[i] = jobs[i].n1 / 2;
-                    swsplit[i] = 1;
+                    swsplit[i] = cand[i];
                     ++nsplit;
-                    nextra += 2;
+                    nextra += cand[i];
                 }
     }
     const int64_t np = njobs + nextra;
@@ -498,10 +504,11 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
             const wf_job &j = jobs[i];
             if (swsplit[i]) {
                 // Right half from a zero boundary (pair r), reversed right
-                // half (pair r + 1); each half keeps its last column.
+                // half (pair r + 1) and, with the running maximum, the
+                // clamped reversed right half (pair r + 2); the left and
+                // the reversed halves keep their last column.
                 const int64_t nr = j.n1 - mid[i];
-                place(ph->pairs[r], j.problem, nr, j.n2, nr);
-                place(ph->pairs[r + 1], j.problem, nr, j.n2, nr);
+                for (int k = 0; k < swsplit[i]; ++k) place(ph->pairs[r + k], j.problem, nr, j.n2, nr);
                 WfSplitDev sp{};
                 sp.job = (int32_t)i;
                 sp.rev = (int32_t)(r + 1);
@@ -516,8 +523,14 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
                 ph->pairs[r + 1].col_off = ph->colw_entries;
                 sp.rev_off = ph->colw_entries;
                 ph->colw_entries += entries;
+                if (swsplit[i] == 3) {
+                    sp.crev = (int32_t)(r + 2);
+                    ph->pairs[r + 2].col_off = ph->colw_entries;
+                    sp.crev_off = ph->colw_entries;
+                    ph->colw_entries += entries;
+                }
                 ph->splits.push_back(sp);
-                r += 2;
+                r += swsplit[i];
                 continue;
             }
             place(ph->pairs[r], j.problem, j.n1 - mid[i], j.n2, j.n1 - mid[i]);
@@ -571,7 +584,13 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
             for (int64_t k = 0; k < j.n2; ++k) rb[k] = fb[j.n2 - 1 - k];
             rp.mode = f.mode;
             rv.mode = (f.mode & (WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD)) | WFK_MODE_NOMAX | WFK_MODE_SWE;
-            if (!(f.mode & WFK_MODE_PERM4) || !(f.mode & WFK_MODE_NOMAX)) return WF_ERR_INVALID;
+            if (!(f.mode & WFK_MODE_PERM4) || !(f.mode & WFK_MODE_NOMAX) != (sp.crev >= 0)) return WF_ERR_INVALID;
+            if (sp.crev >= 0) {
+                WfPairDev &cv = ph->pairs[sp.crev];
+                std::memcpy(ph->codes.data() + cv.a_off, ra, (size_t)nr);
+                std::memcpy(ph->codes.data() + cv.b_off, rb, (size_t)j.n2);
+                cv.mode = (f.mode & (WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD)) | WFK_MODE_NOMAX;
+            }
             continue;
         }
         // Reverse half: a[mid..n1) and b reversed, textbook ED boundary (its

@@ -727,28 +727,52 @@ __device__ void bs_group(const WfKernelArgs &ka, const WfPairDev &pd, int pair, 
 // max(R0, max_y H(y, mid) + that weight), H(0, mid) = 0, E(0) = 0.
 __device__ void sw_split_combine(const WfSplitDev &sp, const uint32_t *colw, WfResultDev *res)
 {
+    // Bottom-right cell: an alignment into (n2, n1) starts right of column
+    // mid (pair f0's cell) or leaves column mid last at some row y after a
+    // best local alignment into (y, mid), H(y, mid), and then takes the best
+    // path from (y, mid) to the corner, E(n2 - y) - 2 (n2 - y + r0) from the
+    // unclamped reverse pair (E(0) = H(0, mid) = 0).  The running maximum
+    // likewise: the halves' own maxima, or H(y, mid) plus the best path from
+    // (y, mid) to any cell right of it, C(n2 - y) from the clamped reverse
+    // pair (C(0) = 0).
     const int n2 = sp.n2, nr = sp.r0;
+    const bool mx = sp.crev >= 0;
     const int *h = reinterpret_cast<const int *>(colw) + 2 * sp.fwd_off;
     const int *e = reinterpret_cast<const int *>(colw) + 2 * sp.rev_off;
-    int best = INT_MIN;
+    const int *c = reinterpret_cast<const int *>(colw) + 2 * sp.crev_off;
+    int best = INT_MIN, bestm = 0;
     for (int y = threadIdx.x; y <= n2; y += blockDim.x) {
         const int i = n2 - y;
         const int hv = y > 0 ? h[y - 1] : 0;
         const int ev = i > 0 ? e[i - 1] : 0;
         best = max(best, hv + ev - 2 * (i + nr));
+        if (mx) bestm = max(bestm, hv + (i > 0 ? c[i - 1] : 0));
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off));
-    __shared__ int wb[16];
+    for (int off = 32; off > 0; off >>= 1) {
+        best = max(best, __shfl_xor(best, off));
+        bestm = max(bestm, __shfl_xor(bestm, off));
+    }
+    __shared__ int wb[16], wm[16];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) wb[wv] = best;
+    if (lane == 0) {
+        wb[wv] = best;
+        wm[wv] = bestm;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int b = wb[0];
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) b = max(b, wb[k]);
+        int b = wb[0], m = wm[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            b = max(b, wb[k]);
+            m = max(m, wm[k]);
+        }
         WfResultDev &r = res[sp.job];
         r.last = max(b, res[sp.f0].last);
         r.status |= res[sp.rev].status | res[sp.f0].status;
+        if (mx) {
+            r.maxs = max(max(m, r.maxs), res[sp.f0].maxs);
+            r.status |= res[sp.crev].status;
+        }
     }
 }
 

@@ -64,10 +64,13 @@ struct WfPairDev {
 // the score is min (ED) / max (LCS) over y of F(y) + R(y) (Hirschberg's
 // split, J. ACM 1975; CACM 18(6) 1975 for LCS).
 //
-// An SW pair (prob WFK_SW, cell engine, bottom-right cell only) splits into
-// three: `job` (columns 1..mid), `f0` (columns mid+1..n1 from a zero
-// boundary) and `rev` (the reversed right half, WFK_MODE_SWE); r0 is the
-// right half's column count (see sw_split_combine in wf_bitpar.hip).
+// An SW pair (prob WFK_SW, cell engine with byte tables) splits into three:
+// `job` (columns 1..mid), `f0` (columns mid+1..n1 from a zero boundary) and
+// `rev` (the reversed right half, WFK_MODE_SWE); r0 is the right half's
+// column count.  A pair that keeps the running maximum adds a fourth, `crev`
+// (the reversed right half with the clamped recurrence, no maximum), whose
+// last column starts at colw entry crev_off (see sw_split_combine in
+// wf_bitpar.hip).
 struct WfSplitDev {
     int32_t job;       // result slot (the forward half's pair)
     int32_t rev;       // the reverse half's pair
@@ -77,6 +80,9 @@ struct WfSplitDev {
     int32_t r0;        // ED: the reverse half's row 0 at its last column (n1 - mid); LCS: 0; SW: n1 - mid
     int64_t fwd_off;   // colw entry offsets of the two halves
     int64_t rev_off;
+    int32_t crev = -1; // SW with the running maximum: the clamped reverse pair; -1: none
+    int32_t pad = 0;
+    int64_t crev_off = 0;
 };
 
 // One unit of persistent work: strip `group` of pair `pair`.  Tasks of a

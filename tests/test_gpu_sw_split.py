@@ -1,12 +1,13 @@
-"""Column-split SW pairs: an SW pair of at least 4096 columns whose output is
-the bottom-right cell only (no_max, the reference's SW()), on the byte
-tables (at most 4 symbols), in a launch with room for three times its
+"""Column-split SW pairs: an SW pair of at least 4096 columns on the byte
+tables (at most 4 symbols), in a launch with room for all of its pairs'
 strips, runs as three pairs at once -- the left columns forward, the right
 columns forward from a zero boundary, the right columns reversed without
-the clamp -- and the bottom-right cell is combined on the device from the
-halves' last columns.  Bit-exact against the oracle, against the unsplit
-engine (WFDP_SPLIT=0) and the unsplit plan that keeps the running maximum,
-with alignments that cross the cut, per-pair launches and graph replays."""
+the clamp -- when its output is the bottom-right cell only (no_max, the
+reference's SW()), and as four with the running maximum (plus the right
+columns reversed with the clamp); the bottom-right cell and the maximum are
+combined on the device from the halves' last columns.  Bit-exact against
+the oracle and the unsplit engine (WFDP_SPLIT=0), with alignments that
+cross the cut, per-pair launches and graph replays."""
 import numpy as np
 import pytest
 
@@ -29,12 +30,12 @@ def _strips(n2, strip_rows=512):
     return (n2 + strip_rows - 1) // strip_rows
 
 
-def _expected_tasks(d, n2):
-    """Strips of the plan described by d: three times the pair's when the
+def _expected_tasks(d, n2, pairs=3):
+    """Strips of the plan described by d: `pairs` times the pair's when the
     chosen kernel has two rows per lane (the byte tables the split needs;
     shorter pairs get one row per lane), once otherwise."""
     strips = _strips(n2, 64 * d["waves"] * d["rows_per_lane"])
-    return 3 * strips if d["rows_per_lane"] == 2 else strips
+    return pairs * strips if d["rows_per_lane"] == 2 else strips
 
 
 @pytest.mark.parametrize("n1,n2", SHAPES)
@@ -54,6 +55,44 @@ def test_sw_split_matches_oracle(ctx, n1, n2):
         assert r.status == 0 and r.last == want and r.max_score == -1
         if n2 >= 512:
             assert d["tasks"] == 3 * _strips(n2)
+
+
+@pytest.mark.parametrize("n1,n2", SHAPES)
+def test_sw_split_with_the_maximum_matches_oracle(ctx, n1, n2):
+    rng = np.random.default_rng(n1 * 19 + n2)
+    for alpha in (2, 4):
+        a, b = _pair(rng, n1, n2, alpha)
+        want = O.score("SW", a, b)
+        plan = ctx.plan([Job("SW", a, b)])
+        d = plan.describe()
+        plan.run()
+        r = plan.results()[0]
+        plan.close()
+        print(f"  SW {n1}x{n2} alpha {alpha} with max: tasks {d['tasks']} got {(r.last, r.max_score)} "
+              f"oracle {want}", flush=True)
+        assert d["engine"] == "async" and d["tasks"] == _expected_tasks(d, n2, 4)
+        assert r.status == 0 and (r.last, r.max_score) == want
+
+
+def test_sw_split_maximum_across_the_cut(ctx):
+    """The best local alignment crosses the middle column, lies left of it,
+    or lies right of it; a repeat on both sides competes."""
+    rng = np.random.default_rng(77)
+    for n1, n2, lo, hi in [(6000, 3000, 2000, 4000), (6000, 3000, 200, 2500), (6000, 3000, 3500, 5800),
+                           (8192, 1024, 4096 - 512, 4096 + 512)]:
+        a, b = _pair(rng, n1, n2, 4)
+        seg = a[lo:hi].copy()
+        seg[::89] = (seg[::89] + 1) % 4
+        k = min(len(seg), n2 - 100)
+        b[50:50 + k] = seg[:k]
+        want = O.score("SW", a, b)
+        plan = ctx.plan([Job("SW", a, b)])
+        assert plan.describe()["tasks"] == 4 * _strips(n2)
+        plan.run()
+        r = plan.results()[0]
+        plan.close()
+        print(f"  planted max {n1}x{n2} [{lo},{hi}): got {(r.last, r.max_score)} oracle {want}", flush=True)
+        assert r.status == 0 and (r.last, r.max_score) == want and want[1] > 1000
 
 
 def test_sw_split_alignment_across_the_cut(ctx):
@@ -84,16 +123,38 @@ def test_sw_split_off_and_with_the_maximum(ctx, monkeypatch):
     p.run()
     assert p.describe()["tasks"] == 3 * _strips(3000) and p.results()[0].last == last
     p.close()
-    # The running maximum needs every cell of the right half: no split.
+    # The running maximum adds the clamped reverse pair.
     p = ctx.plan([Job("SW", a, b)])
     p.run()
     r = p.results()[0]
-    assert p.describe()["tasks"] == _strips(3000) and (r.last, r.max_score) == (last, mx)
+    assert p.describe()["tasks"] == 4 * _strips(3000) and (r.last, r.max_score) == (last, mx)
     p.close()
     monkeypatch.setenv("WFDP_SPLIT", "0")
     p = ctx.plan([Job("SW", a, b, no_max=True)])
     p.run()
     assert p.describe()["tasks"] == _strips(3000) and p.results()[0].last == last
+    p.close()
+    p = ctx.plan([Job("SW", a, b)])
+    p.run()
+    r = p.results()[0]
+    assert p.describe()["tasks"] == _strips(3000) and (r.last, r.max_score) == (last, mx)
+    p.close()
+
+
+def test_sw_split_four_pairs_need_a_cu_per_strip(ctx):
+    """86 strips: four pairs (344 strips) exceed the 256 CUs, so the pair
+    with the running maximum runs unsplit; bit-exact either way."""
+    rng = np.random.default_rng(78)
+    a, b = _pair(rng, 4096, 86 * 512, 4)
+    last, mx = O.score("SW", a, b)
+    p = ctx.plan([Job("SW", a, b)])
+    p.run()
+    r = p.results()[0]
+    assert p.describe()["tasks"] == 86 and (r.last, r.max_score) == (last, mx)
+    p.close()
+    p = ctx.plan([Job("SW", a, b, no_max=True)])
+    p.run()
+    assert p.describe()["tasks"] in (86, 3 * 86) and p.results()[0].last == last
     p.close()
 
 
@@ -108,17 +169,23 @@ def test_sw_split_more_than_four_symbols_is_not_split(ctx):
 
 def test_sw_split_per_pair_launches_and_graph_replays(ctx):
     rng = np.random.default_rng(74)
-    jobs = [Job("SW", *_pair(rng, 4500 + 300 * i, 700 + 250 * i, 4), no_max=True) for i in range(3)]
-    want = [O.score("SW", j.a, j.b)[0] for j in jobs]
+    jobs = [Job("SW", *_pair(rng, 4500 + 300 * i, 700 + 250 * i, 4), no_max=i != 1) for i in range(3)]
+    want = [O.score("SW", j.a, j.b) for j in jobs]
+    want = [(w[0], -1 if i != 1 else w[1]) for i, w in enumerate(want)]
     p = ctx.plan(jobs)
+    assert p.describe()["tasks"] == sum((4 if i == 1 else 3) * _strips(j.b.size) for i, j in enumerate(jobs))
+
+    def got():
+        return [(r.last, r.max_score) for r in p.results()]
+
     p.run()
-    assert [r.last for r in p.results()] == want
+    assert got() == want
     p.run(per_pair=True)
-    assert [r.last for r in p.results()] == want
+    assert got() == want
     p.capture()
     for rep in range(3):
         p.run()
-        assert [r.last for r in p.results()] == want
+        assert got() == want
         # Eager work between replays.
         a, b = _pair(rng, 4200, 300 + rep, 4)
         assert ctx.align("SW", a, b, tile=Tile(no_max=True)).last == O.score("SW", a, b)[0]
@@ -140,6 +207,7 @@ def test_sw_split_tile_engine_flag(ctx):
     a, b = _pair(rng, 5000, 1500, 4)
     want = O.score("SW", a, b)[0]
     for tile, tasks in ((Tile(engine="async", no_max=True), 3 * _strips(1500)),
+                        (Tile(engine="async"), 4 * _strips(1500)),
                         (Tile(engine="lockstep", no_max=True), None)):
         r = ctx.align("SW", a, b, tile=tile)
         assert r.status == 0 and r.last == want
