@@ -376,7 +376,7 @@ constexpr int64_t kSplitMinCols = 1024;
 constexpr int64_t kSwSplitMinCols = 4096;
 
 int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
-                  WfPlanHost *ph)
+                  WfPlanHost *ph, int64_t max_groups)
 {
     int64_t max_rows = 0, total_cells = 0;
     for (int64_t i = 0; i < njobs; ++i) {
@@ -442,8 +442,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         // without it, 384 strips: 3.18 -> 3.03 ms).
         const bool four = std::find(cand.begin(), cand.end(), 3) != cand.end();
         const int64_t cus = ctx ? ctx->prop.multiProcessorCount : 256;
-        const int64_t room = std::min<int64_t>(ctx ? wf_max_resident_groups(ctx, v) : 256,
-                                               four ? cus : INT64_MAX);
+        const int64_t room = std::min({ctx ? (int64_t)wf_max_resident_groups(ctx, v) : 256,
+                                       four ? cus : INT64_MAX, max_groups > 0 ? max_groups : INT64_MAX});
         if (extra > 0 && strips + extra <= room)
             for (int64_t i = 0; i < njobs; ++i)
                 if (cand[i]) {

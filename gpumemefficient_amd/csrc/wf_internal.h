@@ -113,8 +113,11 @@ bool wf_tune_lookup(const char *arch, int cus, int problem, int batch, int bucke
                     int *K);
 int wf_default_waves(int64_t max_rows, int64_t total_cells, int npairs);
 int wf_default_rows_per_lane(int waves, int64_t max_rows, int64_t total_cells, int npairs);
+// max_groups > 0: the launch runs at most that many workgroups (a server
+// slot's share of the chip); an SW column split is planned only if all its
+// strips fit in them.
 int wf_build_plan(struct wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile *tile,
-                  WfPlanHost *ph);
+                  WfPlanHost *ph, int64_t max_groups = 0);
 // Device bytes of every buffer a plan needs.
 struct WfPlanBytes {
     size_t codes, pairs, tasks, bnd, res, counters, epoch, colw, splits;

@@ -250,13 +250,23 @@ static int launch_slot(wf_server *s, Slot &sl, std::vector<Request> &&batch)
         const Request &r = sl.reqs[i];
         jobs[i] = wf_job{r.problem, r.flags, r.a.data(), (int64_t)r.a.size(), r.b.data(), (int64_t)r.b.size()};
     }
-    sl.ph = WfPlanHost();
-    int st = wf_build_plan(s->ctx, jobs.data(), (int64_t)jobs.size(), nullptr, &sl.ph);
-    if (st != WF_OK) return st;
     const int nslots = (int)s->slots.size();
     auto share = [&](const WfVariant *v) {
         return std::max(1, wf_max_resident_groups(s->ctx, v) / std::max(1, nslots));
     };
+    // The variant's share bounds an SW column split (the split's strips must
+    // fit in the launch's workgroups, or the launch would go to the lock-step
+    // engine below).
+    sl.ph = WfPlanHost();
+    int st = wf_build_plan(s->ctx, jobs.data(), (int64_t)jobs.size(), nullptr, &sl.ph);
+    if (st != WF_OK) return st;
+    if (sl.ph.variant->engine == WF_ENGINE_ASYNC && !sl.ph.splits.empty() &&
+        (int64_t)sl.ph.tasks.size() > share(sl.ph.variant)) {
+        const int64_t cap = share(sl.ph.variant);
+        sl.ph = WfPlanHost();
+        if ((st = wf_build_plan(s->ctx, jobs.data(), (int64_t)jobs.size(), nullptr, &sl.ph, cap)) != WF_OK)
+            return st;
+    }
     // The asynchronous engine only pays off when all strips of the launch are
     // resident within the slot's share of the chip; otherwise the lock-step
     // engine's higher occupancy wins.

@@ -204,3 +204,29 @@ def test_server_no_max_requests(resident):
         last, mx = O.score("SW", j.a, j.b)
         assert got[t].status == 0 and got[t].last == last
         assert got[t].max_score == -1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("resident", MODES)
+def test_server_sw_requests_that_column_split(resident):
+    """SW requests of at least 4096 columns on 4 letters, which a launch of
+    their own plans as column-split pairs: alone and together, with and
+    without the maximum, among them one (20000 rows with the maximum) whose
+    four split pairs would exceed a slot's share of the chip.  Bit-exact."""
+    rng = np.random.default_rng(35)
+    shapes = [(4096, 1000, True), (5000, 3000, False), (4096, 20000, False), (8192, 700, False), (6000, 6000, True)]
+    jobs = [Job("SW", rng.integers(0, 4, n1, dtype=np.int32), rng.integers(0, 4, n2, dtype=np.int32), no_max=nm)
+            for n1, n2, nm in shapes]
+    want = [O.score("SW", j.a, j.b) for j in jobs]
+    want = [(w[0], -1 if j.no_max else w[1]) for w, j in zip(want, jobs)]
+    with Server(0, streams=4, resident=resident) as srv:
+        alone = []
+        for j in jobs:
+            srv.submit(j)
+            srv.wait_idle(60)
+            alone += srv.poll()
+        tickets = [srv.submit(j) for j in jobs]
+        srv.wait_idle(120)
+        mix = {c.ticket: c for c in srv.poll()}
+    assert [(c.status, c.last, c.max_score) for c in alone] == [(0,) + w for w in want]
+    assert [(mix[t].status, mix[t].last, mix[t].max_score) for t in tickets] == [(0,) + w for w in want]
