@@ -81,7 +81,21 @@ extern "C" int wf_align_noalloc(wf_ctx *ctx, int problem, const int32_t *a, int6
     if ((st = wf_build_plan(ctx, &j, 1, tile, &ph)) != WF_OK) return st;
     WfPlanBytes nb;
     wf_plan_bytes(&ph, &nb);
-    if (nb.bnd > kBndBytes) return WF_ERR_SCRATCH;
+    auto fits_scratch = [](const WfPlanBytes &b) {
+        size_t small = 256;
+        for (size_t x : {b.counters, b.res, b.pairs, b.tasks, b.splits, b.colw, b.codes})
+            small += (std::max<size_t>(x, 256) + 255) & ~size_t(255);
+        return b.bnd <= kBndBytes && small <= kBndOff;
+    };
+    // A column-split SW pair needs 1.5 to 2 times the boundary rows of the
+    // pair: where the split does not fit and the pair does, it runs unsplit
+    // (a launch of one workgroup leaves no room for a split).
+    if (!fits_scratch(nb) && ph.variant->engine == WF_ENGINE_ASYNC && !ph.splits.empty()) {
+        ph = WfPlanHost();
+        if ((st = wf_build_plan(ctx, &j, 1, tile, &ph, 1)) != WF_OK) return st;
+        wf_plan_bytes(&ph, &nb);
+    }
+    if (!fits_scratch(nb)) return WF_ERR_SCRATCH;
     // Carve the small buffers out of [256, kBndOff), 256-byte aligned.
     WfArenas ar;
     size_t off = 256;

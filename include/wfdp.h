@@ -157,7 +157,7 @@ int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32
  * wf_align().  Pairs whose strip boundaries or codes do not fit return
  * WF_ERR_SCRATCH (roughly: (n2 / strip_rows - 1) * n1 * 8 bytes of boundary
  * rows within 56 MiB, and n1 + n2 bytes of codes within 8 MiB; 16384 x 16384
- * SW fits).  All calls on one GPU share that GPU's array, and calls that
+ * SW fits; an SW pair whose column split would not fit runs unsplit).  All calls on one GPU share that GPU's array, and calls that
  * overlap on one GPU (several threads or contexts) race: the caller
  * serialises them. */
 #define WF_NOALLOC_SCRATCH_BYTES (64LL << 20)

@@ -78,3 +78,22 @@ def test_noalloc_rejects_pairs_beyond_the_scratch_and_keeps_working():
         assert st_ref == _lib.WF_OK and st == _lib.WF_OK and got == ref, (st, got, ref)
     finally:
         lib.wf_ctx_destroy(ctx)
+
+
+def test_noalloc_runs_an_sw_pair_unsplit_where_its_split_does_not_fit():
+    """SW 65536 x 51200, bottom-right cell only: 100 strips of 512 rows, 99
+    boundary rows of 65536 words (51.9 MB) fit the scratch area; the column
+    split that wf_align plans (three pairs, 300 strips) would need 1.5 times
+    that.  wf_align_noalloc runs it unsplit, with the same result."""
+    _lib, lib, ctx = _lib_and_ctx()
+    try:
+        rng = np.random.default_rng(6)
+        a = rng.integers(0, 4, 65536, dtype=np.int32)
+        b = rng.integers(0, 4, 51200, dtype=np.int32)
+        tile = _lib.wf_tile(0, 0, 0, _lib.WF_TILE_NO_MAX)
+        st_ref, ref = _call(lib.wf_align, ctx, "SW", a, b, tile)
+        st, got = _call(lib.wf_align_noalloc, ctx, "SW", a, b, tile)
+        assert st_ref == _lib.WF_OK and st == _lib.WF_OK and got == ref, (st_ref, st, got, ref)
+        assert ref[1] == -1
+    finally:
+        lib.wf_ctx_destroy(ctx)
