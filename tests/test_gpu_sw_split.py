@@ -1,4 +1,4 @@
-"""Column-split SW pairs: an SW pair of at least 4096 columns on the byte
+"""Column-split SW pairs: an SW pair of at least 1024 columns on the byte
 tables (at most 4 symbols), in a launch with room for all of its pairs'
 strips, runs as three pairs at once -- the left columns forward, the right
 columns forward from a zero boundary, the right columns reversed without
@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 
 # (n1, n2): the smallest split width, odd widths, one row, rows around a
 # wave (128) and a strip (512), several strips, wide and short, tall.
-SHAPES = [(4096, 1), (4097, 63), (4100, 128), (4099, 513), (5000, 1000), (8191, 2047), (9000, 4097),
-          (32768, 600), (4096, 20000)]
+SHAPES = [(1024, 1), (1025, 63), (1500, 128), (2047, 513), (3000, 600), (4096, 1), (4097, 63), (4100, 128),
+          (4099, 513), (5000, 1000), (8191, 2047), (9000, 4097), (32768, 600), (4096, 20000), (1024, 20000)]
 
 
 def _pair(rng, n1, n2, alpha):
@@ -54,7 +54,7 @@ def test_sw_split_matches_oracle(ctx, n1, n2):
         assert d["engine"] == "async" and d["tasks"] == _expected_tasks(d, n2)
         assert r.status == 0 and r.last == want and r.max_score == -1
         if n2 >= 512:
-            assert d["tasks"] == 3 * _strips(n2)
+            assert d["rows_per_lane"] == 2 and d["tasks"] == 3 * _strips(n2, 128 * d["waves"])
 
 
 @pytest.mark.parametrize("n1,n2", SHAPES)
@@ -156,6 +156,20 @@ def test_sw_split_four_pairs_need_a_cu_per_strip(ctx):
     p.run()
     assert p.describe()["tasks"] in (86, 3 * 86) and p.results()[0].last == last
     p.close()
+
+
+def test_sw_split_narrower_than_1024_columns_is_not_split(ctx):
+    rng = np.random.default_rng(79)
+    for nm in (True, False):
+        a, b = _pair(rng, 1023, 3000, 4)
+        last, mx = O.score("SW", a, b)
+        p = ctx.plan([Job("SW", a, b, no_max=nm)])
+        p.run()
+        r = p.results()[0]
+        d = p.describe()
+        assert d["tasks"] == _strips(3000, 64 * d["waves"] * d["rows_per_lane"])
+        assert (r.last, r.max_score) == (last, -1 if nm else mx)
+        p.close()
 
 
 def test_sw_split_more_than_four_symbols_is_not_split(ctx):

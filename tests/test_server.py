@@ -209,12 +209,13 @@ def test_server_no_max_requests(resident):
 @pytest.mark.gpu
 @pytest.mark.parametrize("resident", MODES)
 def test_server_sw_requests_that_column_split(resident):
-    """SW requests of at least 4096 columns on 4 letters, which a launch of
+    """SW requests of at least 1024 columns on 4 letters, which a launch of
     their own plans as column-split pairs: alone and together, with and
     without the maximum, among them one (20000 rows with the maximum) whose
     four split pairs would exceed a slot's share of the chip.  Bit-exact."""
     rng = np.random.default_rng(35)
-    shapes = [(4096, 1000, True), (5000, 3000, False), (4096, 20000, False), (8192, 700, False), (6000, 6000, True)]
+    shapes = [(4096, 1000, True), (5000, 3000, False), (4096, 20000, False), (8192, 700, False), (6000, 6000, True),
+              (1024, 900, False), (1500, 1500, True)]
     jobs = [Job("SW", rng.integers(0, 4, n1, dtype=np.int32), rng.integers(0, 4, n2, dtype=np.int32), no_max=nm)
             for n1, n2, nm in shapes]
     want = [O.score("SW", j.a, j.b) for j in jobs]
