@@ -692,9 +692,9 @@ int wf_copy_plan(WfPlanHost *ph, WfArenas *ar, hipStream_t stream)
     return WF_OK;
 }
 
-int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync)
+// Sizes every device buffer of the plan (growing them as needed).
+static int ensure_arenas(WfPlanHost *ph, WfArenas *ar)
 {
-    (void)ctx;
     WfPlanBytes nb;
     wf_plan_bytes(ph, &nb);
     int st;
@@ -707,11 +707,84 @@ int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t 
     // Launch epoch: zero on allocation, so it never matches a zeroed boundary word.
     if ((st = ar->epoch.ensure(nb.epoch, true)) != WF_OK) return st;
     if ((st = ar->colw.ensure(nb.colw)) != WF_OK) return st;
-    if ((st = ar->splits.ensure(nb.splits)) != WF_OK) return st;
+    return ar->splits.ensure(nb.splits);
+}
+
+int wf_upload_plan_async(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, hipStream_t stream, bool sync)
+{
+    (void)ctx;
+    int st;
+    if ((st = ensure_arenas(ph, ar)) != WF_OK) return st;
     if ((st = wf_copy_plan(ph, ar, stream)) != WF_OK) return st;
     if (sync && hipStreamSynchronize(stream) != hipSuccess) return WF_ERR_DEVICE;
     return WF_OK;
 }
+
+namespace {
+
+size_t round256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// The context's pinned block, at least `bytes` long and idle (a call that
+// failed after queueing its copies may have left them in flight).
+int ensure_pinned(wf_ctx *ctx, size_t bytes)
+{
+    if (ctx->pinned_busy) {
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return WF_ERR_DEVICE;
+        ctx->pinned_busy = false;
+    }
+    if (ctx->pinned_cap >= bytes) return WF_OK;
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
+    ctx->pinned_cap = 0;
+    const size_t cap = std::max<size_t>(round256(bytes), (size_t)1 << 20);
+    if (hipHostMalloc((void **)&ctx->pinned, cap, hipHostMallocDefault) != hipSuccess) {
+        ctx->pinned = nullptr;
+        return WF_ERR_NOMEM;
+    }
+    ctx->pinned_cap = cap;
+    return WF_OK;
+}
+
+// Byte offsets of a plan's buffers in the pinned block: codes, splits,
+// pairs, tasks, then the results.
+struct PinnedLayout {
+    size_t codes = 0, splits, pairs, tasks, res, end;
+    explicit PinnedLayout(const WfPlanHost *ph)
+    {
+        splits = round256(ph->codes.size());
+        pairs = splits + round256(ph->splits.size() * sizeof(WfSplitDev));
+        tasks = pairs + round256(ph->pairs.size() * sizeof(WfPairDev));
+        res = tasks + round256(ph->tasks.size() * sizeof(WfTaskDev));
+        end = res + round256((size_t)std::max<int64_t>(1, ph->npairs) * sizeof(WfResultDev));
+    }
+};
+
+// wf_upload_plan through the context's pinned block: the copies are
+// asynchronous DMA from pinned memory, and the stream orders them before
+// the launch, so the host does not wait for them.  The block must not be
+// rewritten before the stream has drained (wf_align_batch synchronises
+// before it returns).
+int upload_pinned(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar)
+{
+    int st;
+    if ((st = ensure_arenas(ph, ar)) != WF_OK) return st;
+    const PinnedLayout L(ph);
+    if ((st = ensure_pinned(ctx, L.end)) != WF_OK) return st;
+    ctx->pinned_busy = true;
+    auto put = [&](size_t off, void *dst, const void *src, size_t n) -> bool {
+        if (!n) return true;
+        std::memcpy(ctx->pinned + off, src, n);
+        return hipMemcpyAsync(dst, ctx->pinned + off, n, hipMemcpyHostToDevice, ctx->stream) == hipSuccess;
+    };
+    if (!put(L.codes, ar->codes.ptr, ph->codes.data(), ph->codes.size()) ||
+        !put(L.splits, ar->splits.ptr, ph->splits.data(), ph->splits.size() * sizeof(WfSplitDev)) ||
+        !put(L.pairs, ar->pairs.ptr, ph->pairs.data(), ph->pairs.size() * sizeof(WfPairDev)) ||
+        !put(L.tasks, ar->tasks.ptr, ph->tasks.data(), ph->tasks.size() * sizeof(WfTaskDev)))
+        return WF_ERR_DEVICE;
+    return WF_OK;
+}
+
+}  // namespace
 
 int wf_max_resident_groups(wf_ctx *ctx, const WfVariant *v)
 {
@@ -805,15 +878,9 @@ int wf_run_plan(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, int mode, double *ker
     return WF_OK;
 }
 
-int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms,
-                            wf_result *out, int64_t n)
+// The caller's results from the device results r of the plan's pairs.
+static int fill_results(const WfPlanHost *ph, const WfResultDev *r, double kernel_ms, wf_result *out, int64_t n)
 {
-    std::vector<WfResultDev> r((size_t)std::max<int64_t>(1, ph->npairs));
-    if (ph->npairs > 0 &&
-        hipMemcpy(r.data(), ar->res.ptr, ph->npairs * sizeof(WfResultDev), hipMemcpyDeviceToHost) !=
-            hipSuccess)
-        return WF_ERR_DEVICE;
-    (void)ctx;
     int worst = WF_OK;
     for (int64_t i = 0; i < std::min(n, ph->njobs); ++i) {
         wf_result &o = out[i];
@@ -831,6 +898,18 @@ int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel
         if (o.status) worst = o.status;
     }
     return worst;
+}
+
+int wf_download_results(wf_ctx *ctx, WfPlanHost *ph, WfArenas *ar, double kernel_ms,
+                            wf_result *out, int64_t n)
+{
+    std::vector<WfResultDev> r((size_t)std::max<int64_t>(1, ph->npairs));
+    if (ph->npairs > 0 &&
+        hipMemcpy(r.data(), ar->res.ptr, ph->npairs * sizeof(WfResultDev), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return WF_ERR_DEVICE;
+    (void)ctx;
+    return fill_results(ph, r.data(), kernel_ms, out, n);
 }
 
 // ----------------------------------------------------------------------------
@@ -902,6 +981,7 @@ void wf_ctx_destroy(wf_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->scratch.release();
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);  // after the stream's synchronisation above
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -941,10 +1021,19 @@ int wf_align_batch(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile
     WfPlanHost ph;
     int st = wf_build_plan(ctx, jobs, njobs, tile, &ph);
     if (st != WF_OK) return st;
-    if ((st = wf_upload_plan(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
+    if ((st = upload_pinned(ctx, &ph, &ctx->scratch)) != WF_OK) return st;
     double ms = 0.0;
     if ((st = wf_run_plan(ctx, &ph, &ctx->scratch, WF_RUN_PERSISTENT, &ms, nullptr)) != WF_OK) return st;
-    return wf_download_results(ctx, &ph, &ctx->scratch, ms, out, njobs);
+    // Results into the pinned block (its upload part is no longer read).
+    const PinnedLayout L(&ph);
+    WfResultDev *r = reinterpret_cast<WfResultDev *>(ctx->pinned + L.res);
+    if (ph.npairs > 0 &&
+        (hipMemcpyAsync(r, ctx->scratch.res.ptr, ph.npairs * sizeof(WfResultDev), hipMemcpyDeviceToHost,
+                        ctx->stream) != hipSuccess ||
+         hipStreamSynchronize(ctx->stream) != hipSuccess))
+        return WF_ERR_DEVICE;
+    ctx->pinned_busy = false;
+    return fill_results(&ph, r, ms, out, njobs);
 }
 
 int wf_align(wf_ctx *ctx, int problem, const int32_t *a, int64_t n1, const int32_t *b, int64_t n2,

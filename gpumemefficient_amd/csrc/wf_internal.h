@@ -72,6 +72,12 @@ struct wf_ctx {
     hipDeviceProp_t prop{};
     double timeout_s = 10.0;
     WfArenas scratch;    // used by wf_align / wf_align_batch
+    // Pinned host block of wf_align / wf_align_batch: stages a plan's codes
+    // and descriptors for upload and receives the results (hipHostMalloc,
+    // grown on demand, freed with the context).
+    uint8_t *pinned = nullptr;
+    size_t pinned_cap = 0;
+    bool pinned_busy = false;  // copies from or to the block may be in flight on `stream`
 };
 
 struct WfPlanHost {
