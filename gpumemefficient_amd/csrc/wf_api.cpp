@@ -401,6 +401,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
     // finish in about half of it.
     std::vector<int64_t> mid(njobs, 0);
     std::vector<uint8_t> swsplit(njobs, 0);
+    std::vector<std::vector<uint8_t>> trial;  // SW split candidates' codes (a then b) and modes
+    std::vector<int32_t> trial_mode;
     int64_t nsplit = 0, nextra = 0;
     if (bitpar && split_enabled()) {
         int64_t strips = 0;
@@ -421,6 +423,8 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         // once: the halves' strip chains run side by side.
         int64_t strips = 0, extra = 0;
         std::vector<uint8_t> cand(njobs, 0);
+        trial.resize(njobs);
+        trial_mode.assign(njobs, 0);
         for (int64_t i = 0; i < njobs; ++i) {
             const wf_job &j = jobs[i];
             if (j.n1 <= 0 || j.n2 <= 0) continue;
@@ -428,11 +432,14 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
             strips += si;
             if (j.problem != WF_SW || j.n1 < kSwSplitMinCols) continue;
             const bool nomax = ((j.flags | (tile ? tile->flags : 0)) & WF_TILE_NO_MAX) != 0;
-            std::vector<uint8_t> ta((size_t)j.n1), tb((size_t)j.n2);
+            // The trial encoding is kept for the plan's codes below.
+            std::vector<uint8_t> t((size_t)(j.n1 + j.n2));
             int32_t mode = 0;
-            if (wf_encode_job(j, tile ? tile->flags : 0, perm4, ta.data(), tb.data(), &mode) != WF_OK ||
+            if (wf_encode_job(j, tile ? tile->flags : 0, perm4, t.data(), t.data() + j.n1, &mode) != WF_OK ||
                 !(mode & WFK_MODE_PERM4))
                 continue;
+            trial[i] = std::move(t);
+            trial_mode[i] = mode;
             cand[i] = nomax ? 2 : 3;  // extra pairs
             extra += cand[i] * si;
         }
@@ -554,9 +561,15 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
         const wf_job &j = jobs[i];
         WfPairDev &p = ph->pairs[i];
         int32_t mode = 0;
-        const int st = wf_encode_job(j, tile ? tile->flags : 0, perm4, ph->codes.data() + p.a_off,
-                                     ph->codes.data() + p.b_off, &mode);
-        if (st != WF_OK) return st;
+        if (i < (int64_t)trial.size() && !trial[i].empty()) {
+            std::memcpy(ph->codes.data() + p.a_off, trial[i].data(), (size_t)j.n1);
+            std::memcpy(ph->codes.data() + p.b_off, trial[i].data() + j.n1, (size_t)j.n2);
+            mode = trial_mode[i];
+        } else {
+            const int st = wf_encode_job(j, tile ? tile->flags : 0, perm4, ph->codes.data() + p.a_off,
+                                         ph->codes.data() + p.b_off, &mode);
+            if (st != WF_OK) return st;
+        }
         // SW pairs on the row tables may skip the running maximum (the cell
         // engines have instances without it).
         if (j.problem == WF_SW && !(mode & WFK_MODE_CMP) && ((j.flags | (tile ? tile->flags : 0)) & WF_TILE_NO_MAX))
