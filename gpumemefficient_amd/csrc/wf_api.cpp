@@ -443,14 +443,17 @@ int wf_build_plan(wf_ctx *ctx, const wf_job *jobs, int64_t njobs, const wf_tile 
             cand[i] = nomax ? 2 : 3;  // extra pairs
             extra += cand[i] * si;
         }
-        // Four pairs (twice the cells) only pay while every strip has a CU
-        // of its own (SW 65536² with the maximum, 512 strips on 256 CUs:
-        // 3.62 -> 4.27 ms); three still pay at two strips per CU (65536²
-        // without it, 384 strips: 3.18 -> 3.03 ms).
+        // A split pays while every strip has a CU of its own.  Beyond that
+        // only three pairs of a launch of one pair still do, whose strips
+        // start staggered along one long chain (SW 65536², 384 strips on
+        // 256 CUs: 3.18 -> 3.03 ms); four pairs do not (with the maximum,
+        // 512 strips: 3.62 -> 4.27 ms), nor a batch, whose strips all run at
+        // once (16 pairs of 4096², 384 strips: 0.27 ms against 0.21).
         const bool four = std::find(cand.begin(), cand.end(), 3) != cand.end();
         const int64_t cus = ctx ? ctx->prop.multiProcessorCount : 256;
         const int64_t room = std::min({ctx ? (int64_t)wf_max_resident_groups(ctx, v) : 256,
-                                       four ? cus : INT64_MAX, max_groups > 0 ? max_groups : INT64_MAX});
+                                       (four || njobs > 1) ? cus : INT64_MAX,
+                                       max_groups > 0 ? max_groups : INT64_MAX});
         if (extra > 0 && strips + extra <= room)
             for (int64_t i = 0; i < njobs; ++i)
                 if (cand[i]) {

@@ -158,6 +158,21 @@ def test_sw_split_four_pairs_need_a_cu_per_strip(ctx):
     p.close()
 
 
+def test_sw_split_batch_beyond_a_strip_per_cu_is_not_split(ctx):
+    """16 pairs of 4096 x 4096 (128 strips): three pairs each would put two
+    strips on a CU, which only a launch of one pair takes; 2 pairs split."""
+    rng = np.random.default_rng(80)
+    for npairs, split in ((16, False), (2, True)):
+        jobs = [Job("SW", *_pair(rng, 4096, 4096, 4), no_max=True) for _ in range(npairs)]
+        p = ctx.plan(jobs)
+        d = p.describe()
+        p.run()
+        got = [r.last for r in p.results()]
+        p.close()
+        assert d["engine"] == "async" and d["tasks"] == (3 if split else 1) * 8 * npairs
+        assert got == [O.score("SW", j.a, j.b)[0] for j in jobs]
+
+
 def test_sw_split_narrower_than_1024_columns_is_not_split(ctx):
     rng = np.random.default_rng(79)
     for nm in (True, False):
