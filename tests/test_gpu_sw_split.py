@@ -244,3 +244,32 @@ def test_sw_split_tile_engine_flag(ctx):
         if tasks is not None:
             assert p.describe()["tasks"] == tasks
         p.close()
+
+
+def test_sw_split_random_pairs(ctx):
+    """40 seeded random pairs of 1024..12000 columns and 1..6000 rows on 2..4
+    letters, with and without the maximum, every fourth with a planted
+    alignment; each a launch of its own, against the oracle."""
+    rng = np.random.default_rng(2025)
+    nsplit = 0
+    for t in range(40):
+        n1 = int(rng.integers(1024, 12001))
+        n2 = int(rng.integers(1, 6001))
+        while n1 * n2 > 20_000_000:
+            n2 //= 2
+        alpha = int(rng.integers(2, 5))
+        a, b = _pair(rng, n1, max(n2, 1), alpha)
+        if t % 4 == 0:
+            k = min(b.size, n1 // 3)
+            lo, off = int(rng.integers(0, n1 - k + 1)), int(rng.integers(0, b.size - k + 1))
+            b[off:off + k] = a[lo:lo + k]
+        nm = bool(t % 2)
+        last, mx = O.score("SW", a, b)
+        p = ctx.plan([Job("SW", a, b, no_max=nm)])
+        d = p.describe()
+        p.run()
+        r = p.results()[0]
+        p.close()
+        nsplit += d["tasks"] > _strips(b.size, 64 * d["waves"] * d["rows_per_lane"])
+        assert r.status == 0 and (r.last, r.max_score) == (last, -1 if nm else mx), (t, n1, b.size, alpha, nm, d)
+    assert nsplit >= 30
