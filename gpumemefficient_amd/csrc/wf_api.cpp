@@ -69,6 +69,13 @@ __attribute__((target("avx2"))) void scale_avx2(const int32_t *v, int64_t n, int
 {
     scale_body(v, n, scale, out);
 }
+__attribute__((always_inline)) inline uint32_t umax_body(const int32_t *v, int64_t n)
+{
+    uint32_t m = 0;
+    for (int64_t i = 0; i < n; ++i) m = std::max(m, (uint32_t)v[i]);
+    return m;
+}
+__attribute__((target("avx2"))) uint32_t umax_avx2(const int32_t *v, int64_t n) { return umax_body(v, n); }
 __attribute__((always_inline)) inline void reverse_body(const uint8_t *src, int64_t n, uint8_t *dst)
 {
     for (int64_t k = 0; k < n; ++k) dst[k] = src[n - 1 - k];
@@ -95,6 +102,11 @@ void scale_codes(const int32_t *v, int64_t n, int scale, uint8_t *out)
         scale_avx2(v, n, scale, out);
     else
         scale_body(v, n, scale, out);
+}
+// The largest symbol read as unsigned (negative symbols count as large).
+uint32_t umax(const int32_t *v, int64_t n)
+{
+    return host_avx2() ? umax_avx2(v, n) : umax_body(v, n);
 }
 // dst[k] = src[n - 1 - k] (the reverse halves of column-split pairs).
 void reverse_codes(const uint8_t *src, int64_t n, uint8_t *dst)
@@ -230,10 +242,7 @@ bool wf_bitpar_eligible(const wf_job *jobs, int64_t njobs)
     for (int64_t i = 0; i < njobs; ++i) {
         const wf_job &j = jobs[i];
         if (j.problem != WF_LCS && j.problem != WF_ED) return false;
-        bool small = true;
-        for (int64_t k = 0; small && k < j.n1; ++k) small = (uint32_t)j.a[k] < 32u;
-        for (int64_t k = 0; small && k < j.n2; ++k) small = (uint32_t)j.b[k] < 32u;
-        if (small) continue;
+        if (std::max(umax(j.a, j.n1), umax(j.b, j.n2)) < 32u) continue;
         std::unordered_map<int32_t, int> seen;
         for (int64_t k = 0; k < j.n1 && seen.size() <= 32; ++k) seen.emplace(j.a[k], 0);
         for (int64_t k = 0; k < j.n2 && seen.size() <= 32; ++k) seen.emplace(j.b[k], 0);
