@@ -76,6 +76,18 @@ __attribute__((always_inline)) inline uint32_t umax_body(const int32_t *v, int64
     return m;
 }
 __attribute__((target("avx2"))) uint32_t umax_avx2(const int32_t *v, int64_t n) { return umax_body(v, n); }
+__attribute__((always_inline)) inline uint8_t bmax_body(const uint8_t *v, int64_t n)
+{
+    uint8_t m = 0;
+    for (int64_t i = 0; i < n; ++i) m = std::max(m, v[i]);
+    return m;
+}
+__attribute__((target("avx2"))) uint8_t bmax_avx2(const uint8_t *v, int64_t n) { return bmax_body(v, n); }
+__attribute__((always_inline)) inline void shr2_body(uint8_t *v, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) v[i] = (uint8_t)(v[i] >> 2);
+}
+__attribute__((target("avx2"))) void shr2_avx2(uint8_t *v, int64_t n) { shr2_body(v, n); }
 __attribute__((always_inline)) inline void reverse_body(const uint8_t *src, int64_t n, uint8_t *dst)
 {
     for (int64_t k = 0; k < n; ++k) dst[k] = src[n - 1 - k];
@@ -107,6 +119,18 @@ void scale_codes(const int32_t *v, int64_t n, int scale, uint8_t *out)
 uint32_t umax(const int32_t *v, int64_t n)
 {
     return host_avx2() ? umax_avx2(v, n) : umax_body(v, n);
+}
+// The largest code, and codes divided by 4 (the byte tables' plain codes).
+uint8_t bmax(const uint8_t *v, int64_t n)
+{
+    return host_avx2() ? bmax_avx2(v, n) : bmax_body(v, n);
+}
+void shr2(uint8_t *v, int64_t n)
+{
+    if (host_avx2())
+        shr2_avx2(v, n);
+    else
+        shr2_body(v, n);
 }
 // dst[k] = src[n - 1 - k] (the reverse halves of column-split pairs).
 void reverse_codes(const uint8_t *src, int64_t n, uint8_t *dst)
@@ -188,12 +212,9 @@ int wf_encode_job(const wf_job &j, int32_t tile_flags, bool perm4, uint8_t *a_ou
     // SW with at most 4 symbols on a cell engine with two rows per lane:
     // byte row tables and plain codes (WFK_MODE_PERM4).
     if (perm4 && j.problem == WF_SW && !(mode & WFK_MODE_CMP)) {
-        uint8_t top = 0;  // a branch-free maximum vectorises (SSE2 has a packed byte max)
-        for (int64_t k = 0; k < j.n1; ++k) top = std::max(top, a_out[k]);
-        for (int64_t k = 0; k < j.n2; ++k) top = std::max(top, b_out[k]);
-        if (top <= 12) {
-            for (int64_t k = 0; k < j.n1; ++k) a_out[k] >>= 2;
-            for (int64_t k = 0; k < j.n2; ++k) b_out[k] >>= 2;
+        if (std::max(bmax(a_out, j.n1), bmax(b_out, j.n2)) <= 12) {
+            shr2(a_out, j.n1);
+            shr2(b_out, j.n2);
             mode = (mode & WFK_MODE_TEXTBOOK) | WFK_MODE_PERM4 | WFK_MODE_FREE_HEAD;
         }
     }
